@@ -1,0 +1,91 @@
+/*
+ * lprec_amd C ABI: log-polar Radon transform (fwd) and back-projection (adj)
+ * on AMD Instinct MI355X (gfx950).
+ *
+ * Plain C types only.  Device buffers are raw pointers to float32 device
+ * memory; `stream` is a hipStream_t passed as void* (NULL = the legacy
+ * default stream).  Every call returns LP_OK (0) or a negative LP_ERR_*
+ * code; lp_last_error() / lp_pre_error() give the message.  No call ever
+ * resets the device.
+ *
+ * Each entry point below cites the reference interface it replaces
+ * (paths relative to the lprec source tree).
+ */
+#ifndef LPREC_AMD_H
+#define LPREC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LP_OK 0
+#define LP_ERR_ARG (-1)      /* invalid argument / shape / state            */
+#define LP_ERR_ALLOC (-2)    /* host or device allocation failed            */
+#define LP_ERR_DEVICE (-3)   /* HIP runtime or kernel launch error          */
+#define LP_ERR_UNSUPPORTED (-4)
+
+typedef struct lp_pre lp_pre;
+typedef struct lp_handle lp_handle;
+
+/* ---- host precompute (no GPU needed) -------------------------------------
+ * Replaces lpTransform.precompute (src/lprec/lpTransform.py:19-31), i.e.
+ * initsgl.create_gl (initsgl.py:25-57), initsfwd.create_fwd
+ * (initsfwd.py:15-66) and initsadj.create_adj (initsadj.py:19-116).
+ * `cor` is the float centre of rotation (used as float to size the padded
+ * grid, truncated to int for the detector shift, as the reference does).
+ * `filter` is one of none/ramp/shepp-logan/cosine/cosine2/hamming/hann/parzen
+ * (case-insensitive; NULL = none).  `want_fwd` = the reference's flg. */
+int lp_pre_create(int N0, int Nproj, int Nslices, double cor, int cubic, const char* filter, int want_fwd,
+                  lp_pre** out);
+/* sizes[0..4] = element counts of glpars, fwdparsi, fwdparamsf, adjparsi,
+ * adjparamsf (the wire format of src/cuda/params.cu:40-145). */
+int lp_pre_sizes(const lp_pre* pre, int64_t* sizes);
+/* Copies the five arrays out (any pointer may be NULL to skip it). */
+int lp_pre_copy(const lp_pre* pre, float* glpars, int32_t* fwdparsi, float* fwdparamsf, int32_t* adjparsi,
+                float* adjparamsf);
+void lp_pre_destroy(lp_pre* pre);
+/* Message of the last failed lp_pre_* / lp_create call on this thread. */
+const char* lp_pre_error(void);
+
+/* ---- operator handle ------------------------------------------------------
+ * lp_create replaces lpRgpu::lpRgpu(size_t params, int gpu)
+ * (src/cuda/lpRgpu.cu:7-39, SWIG src/cuda/lpRgpu.i:16); the host array is
+ * copied, device scratch is sized by Nslices (glpars[6]). */
+int lp_create(const float* glpars, int device, lp_handle** out);
+/* Replaces lpRgpu::initFwd (lpRgpu.cu:61-85; params.cu:57-99). */
+int lp_init_fwd(lp_handle* h, const int32_t* fwdparsi, const float* fwdparamsf);
+/* Replaces lpRgpu::initAdj (lpRgpu.cu:98-133; params.cu:101-145). */
+int lp_init_adj(lp_handle* h, const int32_t* adjparsi, const float* adjparamsf);
+/* Replaces lpRgpu::execFwdManyPtr(size_t Rptr, size_t fptr, int Nslices0, int gpu)
+ * (lpRgpu.cu:184-193): f [ns0][N0][N0] -> R [ns0][Nproj0][N0], device memory.
+ * Requires 1 <= ns0 <= Nslices. */
+int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream);
+/* Replaces lpRgpu::execAdjManyPtr(size_t fptr, size_t Rptr, int Nslices0, int gpu)
+ * (lpRgpu.cu:196-207): R [ns0][Nproj0][N0] -> f [ns0][N0][N0]; applies the FBP
+ * filter when the adjoint tables carry one. */
+int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream);
+/* Replaces lpRgpu::~lpRgpu (lpRgpu.cu:41-58).  Safe on NULL. */
+void lp_destroy(lp_handle* h);
+/* Message of the last failed call on this handle. */
+const char* lp_last_error(const lp_handle* h);
+/* Replaces lpRgpu::printCurrentGPUMemory (params.cu:147-155). */
+int lp_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+
+/* ---- introspection and tuning (no reference counterpart) ---------------- */
+/* info[0..11] = N, N0, Ntheta, Nrho, Nproj(internal), Nproj0, os, Nslices,
+ * cor(int), cubic, has_filter, device bytes held by the handle (clamped to
+ * int64). */
+int lp_info(const lp_handle* h, int64_t* info);
+/* Kernel schedule knobs: key 0 = log-polar rows per row-FFT block (1..16),
+ * key 1 = slices per block in the row/column passes (>=1), key 2 = rows per
+ * prefilter column chunk (32, 64 or 128).  Returns LP_ERR_ARG for bad values. */
+int lp_set_tuning(lp_handle* h, int key, int value);
+int lp_get_tuning(const lp_handle* h, int key, int* value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LPREC_AMD_H */

@@ -1,0 +1,271 @@
+// Workgroup-cooperative FP32 FFTs held in LDS (gfx950, wave64).
+//
+// A line of M complex values lives at buf[line * ldm + i].  Each stage is a
+// self-sorting Stockham radix-R pass done in place: every thread first pulls
+// the inputs of its butterflies into registers, the workgroup synchronises,
+// then the butterflies are evaluated and scattered to their output slots.
+// Radices 2, 3, 4, 5 and 8 have closed-form butterflies; any other factor
+// (7, 11, 43, ...) runs through the generic O(R) per-output stage.  Twiddles
+// come from a table tw[j] = exp(-2 pi i j / M) built on the host in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define LP_MAX_STAGES 16
+
+struct FftPlan {
+    int M;
+    int nst;
+    int radix[LP_MAX_STAGES];
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by -i (dir = -1) or +i (dir = +1)
+__device__ __forceinline__ float2 rot90(float2 a, int dir) {
+    return dir < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int R>
+__device__ __forceinline__ void dft_small(float2* v, int dir);
+
+template <>
+__device__ __forceinline__ void dft_small<2>(float2* v, int) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<4>(float2* v, int dir) {
+    float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    float2 c = cadd(v[1], v[3]), d = rot90(csub(v[1], v[3]), dir);
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<8>(float2* v, int dir) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<4>(e, dir);
+    dft_small<4>(o, dir);
+    const float r = 0.70710678118654752440f;
+    // o[k] *= W8^k, W8 = exp(dir * 2 pi i / 8)
+    float2 w1 = make_float2(r, dir * r);
+    o[1] = cmul(o[1], w1);
+    o[2] = rot90(o[2], dir);
+    o[3] = cmul(o[3], make_float2(-r, dir * r));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <>
+__device__ __forceinline__ void dft_small<3>(float2* v, int dir) {
+    const float c = -0.5f, s = 0.86602540378443864676f * dir;
+    float2 a = v[0], b = v[1], d = v[2];
+    float2 t = cadd(b, d);
+    float2 u = csub(b, d);
+    float2 m = make_float2(a.x + c * t.x, a.y + c * t.y);
+    float2 su = make_float2(-s * u.y, s * u.x);  // i * s * u
+    v[0] = cadd(a, t);
+    v[1] = cadd(m, su);
+    v[2] = csub(m, su);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<5>(float2* v, int dir) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f * dir, s2 = 0.58778525229247312917f * dir;
+    float2 a = v[0];
+    float2 t1 = cadd(v[1], v[4]), u1 = csub(v[1], v[4]);
+    float2 t2 = cadd(v[2], v[3]), u2 = csub(v[2], v[3]);
+    float2 m1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    float2 m2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    // i*(s1 u1 + s2 u2) and i*(s2 u1 - s1 u2)
+    float2 n1 = make_float2(s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y);
+    float2 n2 = make_float2(s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y);
+    float2 in1 = make_float2(-n1.y, n1.x), in2 = make_float2(-n2.y, n2.x);
+    v[0] = make_float2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
+    v[1] = cadd(m1, in1);
+    v[4] = csub(m1, in1);
+    v[2] = cadd(m2, in2);
+    v[3] = csub(m2, in2);
+}
+
+__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int e, int dir) {
+    float2 w = tw[e];
+    return dir < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// One closed-form radix-R Stockham stage over `nlines` lines.  Requires
+// nlines * M <= EPT * blockDim.x.
+template <int R, int EPT>
+__device__ void fft_stage(float2* buf, int ldm, int nlines, int M, int Ns, const float2* __restrict__ tw, int dir) {
+    constexpr int MAXB = (EPT + R - 1) / R;
+    const int nbf = M / R;
+    const int total = nlines * nbf;
+    const int nt = blockDim.x;
+    float2 v[MAXB][R];
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+        int t = threadIdx.x + c * nt;
+        if (t < total) {
+            int b = t / nbf, i = t - b * nbf;
+            const float2* src = buf + b * ldm + i;
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[c][q] = src[q * nbf];
+        }
+    }
+    __syncthreads();
+    const int tstep = M / (Ns * R);
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+        int t = threadIdx.x + c * nt;
+        if (t < total) {
+            int b = t / nbf, i = t - b * nbf;
+            int k = i % Ns;
+            if (Ns > 1) {
+                int base = k * tstep;
+#pragma unroll
+                for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], twiddle(tw, q * base, dir));
+            }
+            dft_small<R>(v[c], dir);
+            float2* dst = buf + b * ldm + (i - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) dst[q * Ns] = v[c][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Generic radix-R stage: each work item produces one output of one butterfly
+// by an R-term sum.  Requires nlines * M <= EPT * blockDim.x.
+template <int EPT>
+__device__ void fft_stage_generic(float2* buf, int ldm, int nlines, int M, int Ns, int R,
+                                  const float2* __restrict__ tw, int dir) {
+    const int nbf = M / R;
+    const int total = nlines * M;
+    const int nt = blockDim.x;
+    const int tstep = M / (Ns * R);
+    float2 out[EPT];
+#pragma unroll
+    for (int c = 0; c < EPT; ++c) {
+        int t = threadIdx.x + c * nt;
+        if (t < total) {
+            int b = t / M, o = t - b * M;
+            int p = o / nbf, i = o - p * nbf;
+            int k = i % Ns;
+            int e = tstep * (k + p * Ns);  // exponent of W_M per input index q
+            const float2* src = buf + b * ldm + i;
+            float2 acc = make_float2(0.f, 0.f);
+            int ex = 0;
+            for (int q = 0; q < R; ++q) {
+                float2 w = twiddle(tw, ex, dir);
+                float2 x = src[q * nbf];
+                acc.x += x.x * w.x - x.y * w.y;
+                acc.y += x.x * w.y + x.y * w.x;
+                ex += e;
+                if (ex >= M) ex -= M;
+            }
+            out[c] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < EPT; ++c) {
+        int t = threadIdx.x + c * nt;
+        if (t < total) {
+            int b = t / M, o = t - b * M;
+            int p = o / nbf, i = o - p * nbf;
+            int k = i % Ns;
+            buf[b * ldm + (i - k) * R + k + p * Ns] = out[c];
+        }
+    }
+    __syncthreads();
+}
+
+// Full in-place FFT (dir = -1 forward, +1 inverse, unnormalised) of nlines
+// lines of length plan.M.  Ends with a barrier.
+template <int EPT>
+__device__ void fft_lds(float2* buf, int ldm, int nlines, const FftPlan& plan, const float2* __restrict__ tw, int dir) {
+    int Ns = 1;
+    const int M = plan.M;
+    for (int s = 0; s < plan.nst; ++s) {
+        int R = plan.radix[s];
+        switch (R) {
+            case 2: fft_stage<2, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
+            case 3: fft_stage<3, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
+            case 4: fft_stage<4, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
+            case 5: fft_stage<5, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
+            case 8: fft_stage<8, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
+            default: fft_stage_generic<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir); break;
+        }
+        Ns *= R;
+    }
+}
+
+// Real-to-complex post-processing.  Line b holds Z = FFT_h(z) with
+// z[n] = x[2n] + i x[2n+1] (N = 2h real samples).  Overwrites it with
+// X[0..h-1] of the length-N real DFT, X[0] (real) and X[h] (real) packed as
+// X[0] + i X[h] in slot 0.  twN[k] = exp(-2 pi i k / N).
+__device__ inline void r2c_post(float2* buf, int ldm, int nlines, int h, const float2* __restrict__ twN) {
+    const int npairs = h / 2 + 1;
+    for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
+        int b = t / npairs, k = t - b * npairs;
+        float2* L = buf + b * ldm;
+        if (k == 0) {
+            float2 z = L[0];
+            L[0] = make_float2(z.x + z.y, z.x - z.y);
+        } else {
+            int kk = h - k;
+            float2 A = L[k], B = L[kk];
+            float2 Bc = cconj(B);
+            float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
+            float2 D = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
+            float2 O = make_float2(D.y, -D.x);  // D / i
+            float2 WO = cmul(twN[k], O);
+            L[k] = cadd(E, WO);
+            if (kk != k) L[kk] = cconj(csub(E, WO));
+        }
+    }
+    __syncthreads();
+}
+
+// Inverse of r2c_post: turns the packed half spectrum X into Z' such that the
+// unnormalised inverse FFT_h of Z' gives N * x (even samples in .x, odd in .y).
+__device__ inline void c2r_pre(float2* buf, int ldm, int nlines, int h, const float2* __restrict__ twN) {
+    const int npairs = h / 2 + 1;
+    for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
+        int b = t / npairs, k = t - b * npairs;
+        float2* L = buf + b * ldm;
+        if (k == 0) {
+            float2 p = L[0];  // X0 + i Xh
+            L[0] = make_float2(p.x + p.y, p.x - p.y);
+        } else {
+            int kk = h - k;
+            float2 P = L[k], Q = L[kk];
+            float2 S = cadd(P, cconj(Q));
+            float2 D = csub(P, cconj(Q));
+            float2 w = twN[k];
+            float2 iwc = make_float2(w.y, w.x);  // i * conj(w)
+            L[k] = cadd(S, cmul(iwc, D));
+            if (kk != k) {
+                // Z'[h-k] = conj(S) + i conj(W^{h-k}) (Q - conj P), conj(W^{h-k}) = -W^k
+                float2 S2 = cconj(S);
+                float2 D2 = cconj(make_float2(-D.x, -D.y));  // Q - conj(P) = -conj(D)
+                float2 imw = make_float2(w.y, -w.x);          // -i * w
+                L[kk] = cadd(S2, cmul(imw, D2));
+            }
+        }
+    }
+    __syncthreads();
+}

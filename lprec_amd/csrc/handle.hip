@@ -1,0 +1,509 @@
+// Operator handle: owns the device tables and scratch of one geometry on one
+// GPU and drives the kernels of kernels.hip.  Replaces the reference's class
+// lpRgpu (src/include/lpRgpu.cuh:6-81, src/cuda/lpRgpu.cu, params.cu,
+// gridStorage.cu) behind the C ABI of include/lprec_amd.h.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fft64.hpp"
+#include "kernels.hpp"
+#include "lp_internal.hpp"
+
+namespace {
+
+thread_local std::string g_create_err;
+
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+FftPlan make_plan(int M) {
+    FftPlan p;
+    p.M = M;
+    p.nst = 0;
+    int n = M;
+    std::vector<int> r;
+    while (n % 8 == 0) { r.push_back(8); n /= 8; }
+    while (n % 4 == 0) { r.push_back(4); n /= 4; }
+    while (n % 2 == 0) { r.push_back(2); n /= 2; }
+    for (int q : {3, 5}) while (n % q == 0) { r.push_back(q); n /= q; }
+    for (int q = 7; (long long)q * q <= n; q += 2) while (n % q == 0) { r.push_back(q); n /= q; }
+    if (n > 1) r.push_back(n);
+    if ((int)r.size() > LP_MAX_STAGES) throw std::runtime_error("FFT length has too many factors");
+    for (size_t i = 0; i < r.size(); ++i) p.radix[i] = r[i];
+    p.nst = (int)r.size();
+    return p;
+}
+
+}  // namespace
+
+struct lp_handle {
+    int device = 0;
+    int N = 0, N0 = 0, Nt = 0, Nr = 0, P = 0, P0 = 0, Ns = 0, cor = 0, os = 1, h = 0;
+    bool cubic = true;
+    bool has_fwd = false, has_adj = false, has_filter = false;
+    int R = 8, zgroup = 1, colchunk = 64;
+    size_t lds_limit = 65536;
+    std::string err;
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+
+    float* d_erho = nullptr;
+    float* d_G = nullptr;
+    float2* d_S = nullptr;
+    std::map<int, float2*> tw;
+    FftPlan plan_h{}, plan_N{}, plan_M{};
+
+    float* d_fc = nullptr;
+    float2* d_cmapF = nullptr;
+    float2* d_p2lp = nullptr;
+    unsigned char* d_spanrow = nullptr;
+    float2* d_fzF = nullptr;
+
+    float* d_Rc = nullptr;
+    float2* d_cmapA = nullptr;
+    int* d_xrow = nullptr;
+    float4* d_xent = nullptr;
+    float2* d_cmapC = nullptr;
+    float2* d_fzA = nullptr;
+    float* d_H2 = nullptr;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+
+    template <class T>
+    int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        size_t nb = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&q, nb);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(LP_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(nb) + " bytes failed: " +
+                                          hipGetErrorString(e));
+        }
+        allocs.push_back(q);
+        bytes += nb;
+        *p = (T*)q;
+        return LP_OK;
+    }
+    template <class T>
+    int upload(T** p, const std::vector<T>& v) {
+        int rc = alloc(p, v.size());
+        if (rc) return rc;
+        if (v.empty()) return LP_OK;
+        hipError_t e = hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(LP_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        return LP_OK;
+    }
+    int twiddles(int M) {
+        if (tw.count(M)) return LP_OK;
+        std::vector<float2> t(M);
+        for (int j = 0; j < M; ++j) {
+            double a = -2.0 * M_PI * (double)j / (double)M;
+            t[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        float2* d = nullptr;
+        int rc = upload(&d, t);
+        if (rc) return rc;
+        tw[M] = d;
+        return LP_OK;
+    }
+    ~lp_handle() {
+        DeviceGuard g(device);
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+int check_launch(lp_handle* h, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return h->fail(LP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return LP_OK;
+}
+
+// kernel spectrum [Nrho][H] interleaved (wire format) -> transposed [H][Nrho]
+// with the C = 1/(Ntheta Nrho) of the reference's mul kernel folded in.
+std::vector<float2> transpose_fz(const float* src, int Nr, int Nt) {
+    int H = Nt / 2 + 1;
+    double sc = 1.0 / ((double)Nt * (double)Nr);
+    std::vector<float2> out((size_t)H * Nr);
+    for (int i = 0; i < Nr; ++i)
+        for (int c = 0; c < H; ++c) {
+            double re = src[2 * ((size_t)i * H + c)], im = src[2 * ((size_t)i * H + c) + 1];
+            out[(size_t)c * Nr + i] = make_float2((float)(re * sc), (float)(im * sc));
+        }
+    return out;
+}
+
+// LDS line stride of the row passes: one complex of padding when it fits
+int row_ldm(const lp_handle* h, int R) {
+    return (size_t)R * (h->h + 1) * sizeof(float2) <= h->lds_limit ? h->h + 1 : h->h;
+}
+
+int pick_rows(lp_handle* h) {
+    // log-polar rows per row-pass block: as many as the LDS and the FFT
+    // register budget allow, up to the tuned value
+    int R = h->R;
+    while (R > 1) {
+        size_t lds = (size_t)R * row_ldm(h, R) * sizeof(float2);
+        long long pts = (long long)R * h->h;
+        if (lds <= h->lds_limit && pts <= 32LL * 1024 && h->Nr % R == 0) break;
+        R /= 2;
+    }
+    return R;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lp_create(const float* g, int device, lp_handle** out) {
+    g_create_err.clear();
+    if (!out || !g) { g_create_err = "null argument"; return LP_ERR_ARG; }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        g_create_err = "no HIP device available";
+        return LP_ERR_DEVICE;
+    }
+    if (device < 0 || device >= ndev) { g_create_err = "device index out of range"; return LP_ERR_ARG; }
+    std::unique_ptr<lp_handle> h(new lp_handle());
+    h->device = device;
+    h->N = (int)g[0]; h->N0 = (int)g[1]; h->Nt = (int)g[2]; h->Nr = (int)g[3];
+    int nspan = (int)g[4];
+    h->P = (int)g[5]; h->Ns = (int)g[6]; h->cor = (int)g[7]; h->os = (int)g[8]; h->cubic = (int)g[9] != 0;
+    if (nspan != 3 || h->N < 16 || h->N % 16 != 0 || h->Nt != h->N || h->Nr != 2 * h->N || h->os < 1 ||
+        h->P % h->os != 0 || h->Ns < 1 || h->N0 < 1 || h->N0 > h->N) {
+        g_create_err = "inconsistent global parameter array";
+        return LP_ERR_ARG;
+    }
+    h->P0 = h->P / h->os;
+    h->h = h->N / 2;
+    int lb = h->N / 2 - h->cor;
+    if (lb < 0 || lb + h->N0 > h->N) { g_create_err = "cor puts the detector outside the padded grid"; return LP_ERR_ARG; }
+    DeviceGuard guard(device);
+    if (!guard.ok) { g_create_err = "hipSetDevice failed"; return LP_ERR_DEVICE; }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        h->lds_limit = std::min<size_t>((size_t)lds, 65536);
+    h->R = 8;
+    h->R = pick_rows(h.get());
+    try {
+        h->plan_h = make_plan(h->h);
+        h->plan_N = make_plan(h->N);
+        h->plan_M = make_plan(h->Nr);
+    } catch (const std::exception& e) {
+        g_create_err = e.what();
+        return LP_ERR_UNSUPPORTED;
+    }
+    if (lpk::ept_for(h->Nr, 1024) < 0 || lpk::ept_for(h->N, 1024) < 0) {
+        g_create_err = "grid too large for the LDS FFT";
+        return LP_ERR_UNSUPPORTED;
+    }
+    int rc;
+    size_t npts = (size_t)h->Nr * h->Nt;
+    std::vector<float> erho(g + 10, g + 10 + npts);
+    if ((rc = h->upload(&h->d_erho, erho)) || (rc = h->alloc(&h->d_G, 3 * (size_t)h->Ns * npts)) ||
+        (rc = h->alloc(&h->d_S, 3 * (size_t)h->Ns * h->h * h->Nr)) || (rc = h->twiddles(h->h)) ||
+        (rc = h->twiddles(h->N)) || (rc = h->twiddles(h->Nr))) {
+        g_create_err = h->err;
+        return rc;
+    }
+    *out = h.release();
+    return LP_OK;
+}
+
+int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
+    if (!h) return LP_ERR_ARG;
+    if (!pi || !pf) return h->fail(LP_ERR_ARG, "null parameter array");
+    DeviceGuard guard(h->device);
+    const int N = h->N, Nt = h->Nt, Nr = h->Nr, P = h->P;
+    size_t npts = (size_t)Nr * Nt;
+    // params.cu:57-99
+    int Npids[3];
+    size_t si = 0, sf = 0;
+    for (int k = 0; k < 3; ++k) Npids[k] = pi[si++];
+    std::vector<const int32_t*> pids(3);
+    for (int k = 0; k < 3; ++k) { pids[k] = pi + si; si += Npids[k]; }
+    int Nc = pi[si++];
+    const int32_t* cids = pi + si;
+    if (Nc < 0 || (size_t)Nc > npts) return h->fail(LP_ERR_ARG, "bad Ncids in fwd parameters");
+    const float *c1[3], *c2[3], *q1[3], *q2[3];
+    for (int k = 0; k < 3; ++k) { c1[k] = pf + sf; sf += Nc; }
+    for (int k = 0; k < 3; ++k) { c2[k] = pf + sf; sf += Nc; }
+    for (int k = 0; k < 3; ++k) { q1[k] = pf + sf; sf += Npids[k]; }
+    for (int k = 0; k < 3; ++k) { q2[k] = pf + sf; sf += Npids[k]; }
+    const float* fz = pf + sf;
+    // dense per-span C->LP maps
+    const float nanv = std::nanf("");
+    std::vector<float2> cmap(3 * npts, make_float2(nanv, nanv));
+    for (int k = 0; k < 3; ++k)
+        for (int t = 0; t < Nc; ++t) {
+            int id = cids[t];
+            if (id < 0 || (size_t)id >= npts) return h->fail(LP_ERR_ARG, "cids index out of range");
+            cmap[k * npts + id] = make_float2(c1[k][t], c2[k][t]);
+        }
+    // LP->polar coordinates per internal sinogram point, plus the span of each row
+    std::vector<float2> p2lp((size_t)P * N, make_float2(nanv, nanv));
+    std::vector<unsigned char> spanrow(P, 255);
+    for (int k = 0; k < 3; ++k)
+        for (int t = 0; t < Npids[k]; ++t) {
+            int id = pids[k][t];
+            if (id < 0 || id >= P * N) return h->fail(LP_ERR_ARG, "pids index out of range");
+            p2lp[id] = make_float2(q1[k][t], q2[k][t]);
+            int row = id / N;
+            if (spanrow[row] != 255 && spanrow[row] != k) return h->fail(LP_ERR_UNSUPPORTED, "a sinogram row spans two spans");
+            spanrow[row] = (unsigned char)k;
+        }
+    for (int a = 0; a < P; a += h->os)
+        if (spanrow[a] == 255) return h->fail(LP_ERR_ARG, "pids do not cover every kept sinogram row");
+    for (int a = 0; a < P; a += h->os)
+        for (int x = 0; x < N; ++x) {
+            float2 v = p2lp[(size_t)a * N + x];
+            if (v.x != v.x) return h->fail(LP_ERR_ARG, "pids do not cover every kept sinogram point");
+        }
+    for (auto& s : spanrow) if (s == 255) s = 0;
+    std::vector<float2> fzT = transpose_fz(fz, Nr, Nt);
+    int rc;
+    if ((rc = h->upload(&h->d_cmapF, cmap)) || (rc = h->upload(&h->d_p2lp, p2lp)) ||
+        (rc = h->upload(&h->d_spanrow, spanrow)) || (rc = h->upload(&h->d_fzF, fzT)) ||
+        (rc = h->alloc(&h->d_fc, (size_t)h->Ns * N * N)))
+        return rc;
+    h->has_fwd = true;
+    return LP_OK;
+}
+
+int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
+    if (!h) return LP_ERR_ARG;
+    if (!pi || !pf) return h->fail(LP_ERR_ARG, "null parameter array");
+    DeviceGuard guard(h->device);
+    const int N = h->N, Nt = h->Nt, Nr = h->Nr;
+    size_t npts = (size_t)Nr * Nt;
+    // params.cu:101-145
+    int Nc = pi[0], Nl = pi[1], Nw = pi[2];
+    if (Nc < 0 || Nl < 0 || Nw < 0 || (size_t)Nc > (size_t)N * N || (size_t)Nl > npts || (size_t)Nw > npts)
+        return h->fail(LP_ERR_ARG, "bad counts in adj parameters");
+    const int32_t* cids = pi + 3;
+    const int32_t* lpids = cids + Nc;
+    const int32_t* wids = lpids + Nl;
+    int flag = wids[Nw];
+    size_t sf = 0;
+    const float *C1[3], *C2[3], *L1[3], *L2[3], *W1[3], *W2[3];
+    for (int k = 0; k < 3; ++k) { C1[k] = pf + sf; sf += Nc; }
+    for (int k = 0; k < 3; ++k) { C2[k] = pf + sf; sf += Nc; }
+    for (int k = 0; k < 3; ++k) { L1[k] = pf + sf; sf += Nl; }
+    for (int k = 0; k < 3; ++k) { L2[k] = pf + sf; sf += Nl; }
+    for (int k = 0; k < 3; ++k) { W1[k] = pf + sf; sf += Nw; }
+    for (int k = 0; k < 3; ++k) { W2[k] = pf + sf; sf += Nw; }
+    const float* fz = pf + sf;
+    sf += 2 * (size_t)(Nt / 2 + 1) * Nr;
+    const float* filt = flag == 1 ? pf + sf : nullptr;
+    const float nanv = std::nanf("");
+    // polar->LP maps: texture x = s index (lp2p2), y = angle index (lp2p1), lpRgpu.cu:307-311
+    std::vector<float2> cmap(3 * npts, make_float2(nanv, nanv));
+    std::vector<std::vector<float4>> extra(3 * (size_t)Nr);
+    std::vector<unsigned char> used(npts, 0);
+    for (int t = 0; t < Nl; ++t) {
+        int id = lpids[t];
+        if (id < 0 || (size_t)id >= npts) return h->fail(LP_ERR_ARG, "lpids index out of range");
+        for (int k = 0; k < 3; ++k) {
+            if (!used[id]) cmap[k * npts + id] = make_float2(L2[k][t], L1[k][t]);
+            else extra[k * Nr + id / Nt].push_back(make_float4((float)(id % Nt), L2[k][t], L1[k][t], 0.f));
+        }
+        used[id] = 1;
+    }
+    for (int t = 0; t < Nw; ++t) {
+        int id = wids[t];
+        if (id < 0 || (size_t)id >= npts) return h->fail(LP_ERR_ARG, "wids index out of range");
+        for (int k = 0; k < 3; ++k) {
+            if (!used[id]) cmap[k * npts + id] = make_float2(W2[k][t], W1[k][t]);
+            else extra[k * Nr + id / Nt].push_back(make_float4((float)(id % Nt), W2[k][t], W1[k][t], 0.f));
+        }
+        used[id] = 1;
+    }
+    std::vector<int> xrow;
+    std::vector<float4> xent;
+    bool any_extra = false;
+    for (auto& e : extra) if (!e.empty()) any_extra = true;
+    if (any_extra) {
+        xrow.resize(3 * (size_t)(Nr + 1));
+        for (int k = 0; k < 3; ++k) {
+            xrow[k * (Nr + 1)] = (int)xent.size();
+            for (int r = 0; r < Nr; ++r) {
+                auto& e = extra[k * Nr + r];
+                xent.insert(xent.end(), e.begin(), e.end());
+                xrow[k * (Nr + 1) + r + 1] = (int)xent.size();
+            }
+        }
+    }
+    // LP->Cartesian maps over the full N x N grid (lpRgpu.cu:328)
+    size_t NN = (size_t)N * N;
+    std::vector<float2> cmapC(3 * NN, make_float2(nanv, nanv));
+    for (int t = 0; t < Nc; ++t) {
+        int id = cids[t];
+        if (id < 0 || (size_t)id >= NN) return h->fail(LP_ERR_ARG, "adj cids index out of range");
+        for (int k = 0; k < 3; ++k) cmapC[k * NN + id] = make_float2(C1[k][t], C2[k][t]);
+    }
+    std::vector<float2> fzT = transpose_fz(fz, Nr, Nt);
+    int rc;
+    if ((rc = h->upload(&h->d_cmapA, cmap)) || (rc = h->upload(&h->d_cmapC, cmapC)) ||
+        (rc = h->upload(&h->d_fzA, fzT)) || (rc = h->alloc(&h->d_Rc, (size_t)h->Ns * h->P * N)))
+        return rc;
+    if (any_extra && ((rc = h->upload(&h->d_xrow, xrow)) || (rc = h->upload(&h->d_xent, xent)))) return rc;
+    if (filt) {
+        // kernel of the reference filter: h4 = IFFT_4N(ifftshift(filter)) / 4N; keep lags |l| < N
+        int M4 = 4 * N, M2 = 2 * N;
+        std::vector<std::complex<double>> a(M4);
+        for (int k = 0; k < M4; ++k) a[k] = (double)filt[(k + M4 / 2) % M4];
+        lpfft::Plan inv4(M4, +1);
+        inv4.exec(a);
+        std::vector<std::complex<double>> b(M2, 0.0);
+        for (int l = -(N - 1); l <= N - 1; ++l) b[(l + M2) % M2] = a[(l + M4) % M4].real() / M4;
+        lpfft::Plan fwd2(M2, -1);
+        fwd2.exec(b);
+        std::vector<float> H2(N + 1);
+        for (int k = 0; k <= N; ++k) H2[k] = (float)(b[k].real() / (double)M2);
+        if ((rc = h->upload(&h->d_H2, H2)) || (rc = h->twiddles(2 * N))) return rc;
+        h->has_filter = true;
+    }
+    h->has_adj = true;
+    return LP_OK;
+}
+
+static lpk::RowPass row_pass(lp_handle* h, int nz) {
+    lpk::RowPass rp;
+    rp.Ntheta = h->Nt; rp.Nrho = h->Nr; rp.h = h->h; rp.R = h->R;
+    rp.ldm = row_ldm(h, h->R);
+    rp.nz = nz;
+    rp.zgroup = std::max(1, std::min(h->zgroup, nz));
+    rp.planh = h->plan_h;
+    rp.twh = h->tw[h->h];
+    rp.twN = h->tw[h->N];
+    rp.S = h->d_S;
+    return rp;
+}
+
+int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
+    if (!h) return LP_ERR_ARG;
+    if (!h->has_fwd) return h->fail(LP_ERR_ARG, "forward tables not initialised (lp_init_fwd)");
+    if (!R || !f) return h->fail(LP_ERR_ARG, "null data pointer");
+    if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
+    DeviceGuard guard(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    const int N = h->N;
+    lpk::launch_image_rows(f, h->d_fc, ns0, h->N0, N, h->cubic, s);
+    if (h->cubic) lpk::launch_col_iir(h->d_fc, ns0, N, N, h->colchunk, s);
+    lpk::RowPass rp = row_pass(h, ns0);
+    lpk::Src src{h->d_fc, N, N, (long long)N * N};
+    lpk::launch_rows_fwd(rp, src, h->d_cmapF, h->d_erho, nullptr, nullptr, h->cubic, s);
+    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
+    lpk::launch_rows_inv(rp, h->d_G, s);
+    lpk::launch_polar_out(h->d_G, R, ns0, h->Nt, h->Nr, N, h->N0, h->P0, h->os, N / 2 - h->cor, h->d_p2lp,
+                          h->d_spanrow, h->cubic, s);
+    return check_launch(h, "lp_fwd launch");
+}
+
+int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
+    if (!h) return LP_ERR_ARG;
+    if (!h->has_adj) return h->fail(LP_ERR_ARG, "adjoint tables not initialised (lp_init_adj)");
+    if (!R || !f) return h->fail(LP_ERR_ARG, "null data pointer");
+    if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
+    DeviceGuard guard(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    const int N = h->N;
+    lpk::SinoPrep sp;
+    sp.N0 = h->N0; sp.N = N; sp.P0 = h->P0; sp.P = h->P; sp.os = h->os; sp.lb = N / 2 - h->cor;
+    sp.H2 = h->has_filter ? h->d_H2 : nullptr;
+    sp.planN = h->plan_N;
+    sp.twN = h->tw[N];
+    sp.tw2N = h->has_filter ? h->tw[2 * N] : h->tw[N];
+    lpk::launch_sino_rows(R, h->d_Rc, ns0, sp, h->cubic, s);
+    if (h->cubic) lpk::launch_col_iir(h->d_Rc, ns0, h->P, N, h->colchunk, s);
+    lpk::RowPass rp = row_pass(h, ns0);
+    lpk::Src src{h->d_Rc, N, h->P, (long long)h->P * N};
+    lpk::launch_rows_fwd(rp, src, h->d_cmapA, nullptr, h->d_xrow, h->d_xent, h->cubic, s);
+    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
+    lpk::launch_rows_inv(rp, h->d_G, s);
+    lpk::launch_cart_out(h->d_G, f, ns0, h->Nt, h->Nr, N, h->N0, h->d_cmapC, h->cubic, s);
+    return check_launch(h, "lp_adj launch");
+}
+
+void lp_destroy(lp_handle* h) { delete h; }
+
+const char* lp_last_error(const lp_handle* h) {
+    if (!h) return g_create_err.c_str();
+    return h->err.c_str();
+}
+
+int lp_mem_info(int device, size_t* free_bytes, size_t* total_bytes) {
+    DeviceGuard guard(device);
+    if (!guard.ok) return LP_ERR_DEVICE;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return LP_ERR_DEVICE;
+    if (free_bytes) *free_bytes = fr;
+    if (total_bytes) *total_bytes = tot;
+    return LP_OK;
+}
+
+int lp_info(const lp_handle* h, int64_t* info) {
+    if (!h || !info) return LP_ERR_ARG;
+    int64_t v[12] = {h->N, h->N0, h->Nt, h->Nr, h->P, h->P0, h->os, h->Ns, h->cor, h->cubic ? 1 : 0,
+                     h->has_filter ? 1 : 0, (int64_t)h->bytes};
+    std::memcpy(info, v, sizeof(v));
+    return LP_OK;
+}
+
+int lp_set_tuning(lp_handle* h, int key, int value) {
+    if (!h) return LP_ERR_ARG;
+    switch (key) {
+        case 0: {
+            if (value < 1 || value > 16 || (value & (value - 1))) return h->fail(LP_ERR_ARG, "rows per block must be a power of two in [1, 16]");
+            int old = h->R;
+            h->R = value;
+            int r = pick_rows(h);
+            if (r != value) { h->R = old; return h->fail(LP_ERR_ARG, "rows per block exceeds the LDS or FFT budget"); }
+            return LP_OK;
+        }
+        case 1:
+            if (value < 1) return h->fail(LP_ERR_ARG, "slices per block must be >= 1");
+            h->zgroup = value;
+            return LP_OK;
+        case 2:
+            if (value != 32 && value != 64 && value != 128) return h->fail(LP_ERR_ARG, "column chunk must be 32, 64 or 128");
+            h->colchunk = value;
+            return LP_OK;
+    }
+    return h->fail(LP_ERR_ARG, "unknown tuning key");
+}
+
+int lp_get_tuning(const lp_handle* h, int key, int* value) {
+    if (!h || !value) return LP_ERR_ARG;
+    switch (key) {
+        case 0: *value = h->R; return LP_OK;
+        case 1: *value = h->zgroup; return LP_OK;
+        case 2: *value = h->colchunk; return LP_OK;
+    }
+    return LP_ERR_ARG;
+}
+
+}  // extern "C"
