@@ -1,0 +1,209 @@
+"""Benchmark: log-polar Radon fwd+adj throughput on one MI355X.
+
+A step = one batch of ``--slices`` slices through fwd (lpTransform.fwdp) and
+then adj (lpTransform.adjp), inputs resident in HBM.  Workload: N0 = 2048,
+Nproj = 1500 angles on [0, pi), centre N0/2, cubic interpolation, no filter,
+32 slices per batch (the reference's headline fwd+adj configuration), seeded
+random-ellipse phantoms.  Prints one JSON line.
+
+Also reported: the plain path (same kernels, one slice per call), the
+per-stage durations from HIP events recorded on the launch stream during the
+timed region, the roofline of the dominant stage and of the whole pipeline,
+and the FP64 numpy/scipy oracle mirror timed on this host as the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "fwd+adj slices/s at N=2048,1500 proj: batched vs plain GPU; % of HBM roofline"
+
+
+def stage_bytes(g):
+    """Algorithmic HBM bytes per slice of each stage (one launch covers all
+    slices of a batch and all three spans).  L = 4*Nrho*Ntheta bytes is one
+    real log-polar grid; the half spectra S of a span also occupy L bytes."""
+    N0, N, P0, P = g["N0"], g["N"], g["Nproj0"], g["Nproj"]
+    L = 4 * g["Nrho"] * g["Ntheta"]
+    fwd = {
+        "prep_rows": 4 * N0 * N0 + 4 * N * N,      # read f, write padded x-filtered image
+        "col_iir": 8 * N * N,                      # read + write the image
+        "rows_fwd": 4 * N * N + 3 * L,             # image gathered once, S written per span
+        "cols": 3 * 2 * L,                         # S read + written per span
+        "rows_inv": 3 * 2 * L,                     # S read, G written per span
+        "polar_out": 3 * L // 2 + 4 * N0 * P0,     # half of each G (|theta| < beta/2) read, R written
+    }
+    adj = {
+        "sino_rows": 4 * N0 * P0 + 4 * N * P,
+        "col_iir": 8 * N * P,
+        "rows_fwd": 4 * N * P + 3 * L,
+        "cols": 3 * 2 * L,
+        "rows_inv": 3 * 2 * L,
+        "cart_out": 3 * L + 4 * N0 * N0,
+    }
+    return fwd, adj
+
+
+def pipeline_bytes(g):
+    """B_alg per slice of fwd+adj (compulsory traffic of the fused design):
+    B_fwd = 4N0^2 + 16N^2 + 12L + 4N0P0, B_adj = 4N0P0 + 16NP + 12L + 4N0^2."""
+    N0, N, P0, P = g["N0"], g["N"], g["Nproj0"], g["Nproj"]
+    L = 4 * g["Nrho"] * g["Ntheta"]
+    return (4 * N0 * N0 + 16 * N * N + 12 * L + 4 * N0 * P0) + (4 * N0 * P0 + 16 * N * P + 12 * L + 4 * N0 * N0)
+
+
+def cpu_baseline(lp, f0, budget_s):
+    """Time the FP64 oracle mirror (numpy/scipy) on one slice of the workload."""
+    import numpy as np
+    from oracle import mirror
+    threads = len(os.sched_getaffinity(0))
+    tab = mirror.Tables(lp.glpars, lp.fwdparsi, lp.fwdparamsf, lp.adjparsi, lp.adjparamsf)
+    f = np.asarray(f0[:1], dtype=np.float64)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        R = mirror.fwd(tab, f)
+        mirror.adj(tab, R)
+        n += 1
+        el = time.perf_counter() - t0
+        if el + el / n > budget_s or n >= 3:
+            break
+    dt = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": n / dt, "unit": "slices/s", "cores": threads, "kind": "port",
+            "sample": "%d slice(s) fwd+adj of the same workload through the FP64 numpy/scipy mirror "
+                      "(scipy.fft workers=%d), %.1f s, host CPU: %s" % (n, threads, dt, cpu)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--slices", type=int, default=32)
+    ap.add_argument("--N", type=int, default=2048)
+    ap.add_argument("--nproj", type=int, default=1500)
+    ap.add_argument("--plain-slices", type=int, default=8)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(slices shard with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from lprec_amd import lpTransform
+    from lprec_amd.phantom import phantom_stack
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    N0, P0, ns = args.N, args.nproj, args.slices
+    cor = N0 / 2
+    lp = lpTransform(N0, P0, ns, "none", cor, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    h = lp.clphandle[0]
+    g = h.info()
+    f_host = phantom_stack(ns, N0, 12, seed=2048)
+    f = torch.from_numpy(f_host).cuda()
+    R = torch.zeros((ns, P0, N0), device="cuda")
+    back = torch.zeros((ns, N0, N0), device="cuda")
+
+    def step():
+        lp.fwdp(R, f, 0)
+        lp.adjp(back, R, 0)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    h.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fwd_ms, nf = h.profile_read(0)
+    adj_ms, na = h.profile_read(1)
+    h.profile(False)
+    value = ns * args.steps / dt
+
+    # plain path: same kernels, one slice per call, tables re-read every call
+    lp1 = lpTransform(N0, P0, 1, "none", cor, "cubic")
+    lp1.precompute(1)
+    lp1.initcmem(1, 0)
+    nplain = min(args.plain_slices, ns)
+    fs = [f[i:i + 1].contiguous() for i in range(nplain)]
+    R1 = torch.zeros((1, P0, N0), device="cuda")
+    b1 = torch.zeros((1, N0, N0), device="cuda")
+    for x in fs[:2]:
+        lp1.fwdp(R1, x, 0)
+        lp1.adjp(b1, R1, 0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for x in fs:
+        lp1.fwdp(R1, x, 0)
+        lp1.adjp(b1, R1, 0)
+    torch.cuda.synchronize()
+    plain = nplain / (time.perf_counter() - t1)
+
+    # roofline of the dominant stage (largest summed duration)
+    sb_fwd, sb_adj = stage_bytes(g)
+    stages = {}
+    for name, ms in fwd_ms.items():
+        stages["fwd." + name] = (ms / max(nf, 1), sb_fwd[name] * ns)
+    for name, ms in adj_ms.items():
+        stages["adj." + name] = (ms / max(na, 1), sb_adj[name] * ns)
+    dom = max(stages, key=lambda k: stages[k][0])
+    dom_ms, dom_bytes = stages[dom]
+    achieved = dom_bytes / (dom_ms * 1e-3)
+    b_alg = pipeline_bytes(g)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded random-ellipse phantoms (12 ellipses per slice)",
+        "config": {"workload": "lprec fwd+adj, N0=2048, Nproj=1500 on [0,pi), centre N0/2, cubic, no filter, "
+                               "32 slices per batch (internal N=%d, os=%d, Nproj=%d, log-polar %dx%d)"
+                               % (g["N"], g["os"], g["Nproj"], g["Nrho"], g["Ntheta"]),
+                   "N0": N0, "Nproj": P0, "slices_per_batch": ns, "interp": "cubic", "filter": "none"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "alg_bytes_per_launch": int(dom_bytes), "avg_launch_ms": round(dom_ms, 4)},
+        "pipeline_roofline": {"B_alg_bytes_per_slice": int(b_alg),
+                              "achieved_GBps": round(b_alg * value / 1e9, 1),
+                              "frac": round(b_alg * value / HBM_PEAK, 4)},
+        "plain_path": {"value": round(plain, 2), "unit": "slices/s", "batched_over_plain": round(value / plain, 3)},
+        "stage_ms_per_step": {k: round(v[0], 4) for k, v in stages.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(lp, f_host, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

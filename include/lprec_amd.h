@@ -85,6 +85,19 @@ int lp_info(const lp_handle* h, int64_t* info);
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 
+/* Stage timing (no reference counterpart; the reference only prints
+ * wall-clock tic/toc, tests/timing.py).  While enabled, lp_fwd / lp_adj record
+ * a HIP event on the call's stream between consecutive kernel launches.
+ * lp_profile(h, 1) enables and clears, lp_profile(h, 0) disables.
+ * lp_profile_read synchronises the recorded events and returns, for
+ * which = 0 (fwd) or 1 (adj), the summed milliseconds of each stage over all
+ * calls since enabling in ms[0..5], and the call count in *ncalls.
+ * Stages: fwd = prep rows, column prefilter, row pass, column pass, inverse
+ * row pass, polar gather; adj = sinogram rows, column prefilter, row pass,
+ * column pass, inverse row pass, Cartesian gather. */
+int lp_profile(lp_handle* h, int enable);
+int lp_profile_read(lp_handle* h, int which, float* ms, int* ncalls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@ EXPORTS = (
     "lp_pre_create", "lp_pre_sizes", "lp_pre_copy", "lp_pre_destroy", "lp_pre_error",
     "lp_create", "lp_init_fwd", "lp_init_adj", "lp_fwd", "lp_adj", "lp_destroy",
     "lp_last_error", "lp_mem_info", "lp_info", "lp_set_tuning", "lp_get_tuning",
+    "lp_profile", "lp_profile_read",
 )
 
 
@@ -64,6 +65,8 @@ def load():
         "lp_info": (i32, [vp, i64p]),
         "lp_set_tuning": (i32, [vp, i32, i32]),
         "lp_get_tuning": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int)]),
+        "lp_profile": (i32, [vp, i32]),
+        "lp_profile_read": (i32, [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -156,6 +159,20 @@ class Handle:
         v = ctypes.c_int()
         self._check(self.lib.lp_get_tuning(self.h, int(key), ctypes.byref(v)))
         return v.value
+
+    STAGES = {0: ("prep_rows", "col_iir", "rows_fwd", "cols", "rows_inv", "polar_out"),
+              1: ("sino_rows", "col_iir", "rows_fwd", "cols", "rows_inv", "cart_out")}
+
+    def profile(self, enable):
+        """Enable (and clear) or disable per-stage HIP-event timing."""
+        self._check(self.lib.lp_profile(self.h, int(bool(enable))))
+
+    def profile_read(self, which):
+        """-> (dict stage -> summed ms, number of calls) for which = 0 fwd / 1 adj."""
+        ms = (ctypes.c_float * 6)()
+        n = ctypes.c_int()
+        self._check(self.lib.lp_profile_read(self.h, int(which), ms, ctypes.byref(n)))
+        return dict(zip(self.STAGES[which], list(ms))), n.value
 
     def close(self):
         if self.h:

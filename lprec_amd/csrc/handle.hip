@@ -82,6 +82,32 @@ struct lp_handle {
     float2* d_fzA = nullptr;
     float* d_H2 = nullptr;
 
+    // stage timing: one set of kNStage+1 events per profiled call
+    static constexpr int kNStage = 6;
+    bool prof = false;
+    std::vector<std::vector<hipEvent_t>> prof_sets[2];
+    size_t prof_used[2] = {0, 0};
+
+    hipEvent_t* prof_begin(int which) {
+        if (!prof) return nullptr;
+        auto& sets = prof_sets[which];
+        if (prof_used[which] == sets.size()) {
+            std::vector<hipEvent_t> ev(kNStage + 1);
+            for (auto& e : ev)
+                if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            sets.push_back(ev);
+        }
+        return sets[prof_used[which]++].data();
+    }
+    void prof_clear() {
+        for (int w = 0; w < 2; ++w) {
+            for (auto& set : prof_sets[w])
+                for (auto& e : set) (void)hipEventDestroy(e);
+            prof_sets[w].clear();
+            prof_used[w] = 0;
+        }
+    }
+
     int fail(int code, const std::string& msg) {
         err = msg;
         return code;
@@ -126,6 +152,7 @@ struct lp_handle {
     }
     ~lp_handle() {
         DeviceGuard g(device);
+        prof_clear();
         for (void* p : allocs) (void)hipFree(p);
     }
 };
@@ -410,15 +437,24 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     (void)hipGetLastError();
     const int N = h->N;
+    hipEvent_t* ev = h->prof_begin(0);
+    auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
+    mark(0);
     lpk::launch_image_rows(f, h->d_fc, ns0, h->N0, N, h->cubic, s);
+    mark(1);
     if (h->cubic) lpk::launch_col_iir(h->d_fc, ns0, N, N, h->colchunk, s);
+    mark(2);
     lpk::RowPass rp = row_pass(h, ns0);
     lpk::Src src{h->d_fc, N, N, (long long)N * N};
     lpk::launch_rows_fwd(rp, src, h->d_cmapF, h->d_erho, nullptr, nullptr, h->cubic, s);
+    mark(3);
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
+    mark(4);
     lpk::launch_rows_inv(rp, h->d_G, s);
+    mark(5);
     lpk::launch_polar_out(h->d_G, R, ns0, h->Nt, h->Nr, N, h->N0, h->P0, h->os, N / 2 - h->cor, h->d_p2lp,
                           h->d_spanrow, h->cubic, s);
+    mark(6);
     return check_launch(h, "lp_fwd launch");
 }
 
@@ -437,14 +473,23 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     sp.planN = h->plan_N;
     sp.twN = h->tw[N];
     sp.tw2N = h->has_filter ? h->tw[2 * N] : h->tw[N];
+    hipEvent_t* ev = h->prof_begin(1);
+    auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
+    mark(0);
     lpk::launch_sino_rows(R, h->d_Rc, ns0, sp, h->cubic, s);
+    mark(1);
     if (h->cubic) lpk::launch_col_iir(h->d_Rc, ns0, h->P, N, h->colchunk, s);
+    mark(2);
     lpk::RowPass rp = row_pass(h, ns0);
     lpk::Src src{h->d_Rc, N, h->P, (long long)h->P * N};
     lpk::launch_rows_fwd(rp, src, h->d_cmapA, nullptr, h->d_xrow, h->d_xent, h->cubic, s);
+    mark(3);
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
+    mark(4);
     lpk::launch_rows_inv(rp, h->d_G, s);
+    mark(5);
     lpk::launch_cart_out(h->d_G, f, ns0, h->Nt, h->Nr, N, h->N0, h->d_cmapC, h->cubic, s);
+    mark(6);
     return check_launch(h, "lp_adj launch");
 }
 
@@ -494,6 +539,34 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             return LP_OK;
     }
     return h->fail(LP_ERR_ARG, "unknown tuning key");
+}
+
+int lp_profile(lp_handle* h, int enable) {
+    if (!h) return LP_ERR_ARG;
+    DeviceGuard guard(h->device);
+    h->prof_clear();
+    h->prof = enable != 0;
+    return LP_OK;
+}
+
+int lp_profile_read(lp_handle* h, int which, float* ms, int* ncalls) {
+    if (!h || !ms || (which != 0 && which != 1)) return LP_ERR_ARG;
+    DeviceGuard guard(h->device);
+    for (int i = 0; i < lp_handle::kNStage; ++i) ms[i] = 0.f;
+    size_t n = h->prof_used[which];
+    for (size_t c = 0; c < n; ++c) {
+        auto& ev = h->prof_sets[which][c];
+        if (hipEventSynchronize(ev[lp_handle::kNStage]) != hipSuccess)
+            return h->fail(LP_ERR_DEVICE, "hipEventSynchronize failed");
+        for (int i = 0; i < lp_handle::kNStage; ++i) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) != hipSuccess)
+                return h->fail(LP_ERR_DEVICE, "hipEventElapsedTime failed");
+            ms[i] += t;
+        }
+    }
+    if (ncalls) *ncalls = (int)n;
+    return LP_OK;
 }
 
 int lp_get_tuning(const lp_handle* h, int key, int* value) {

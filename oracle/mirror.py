@@ -24,7 +24,12 @@ against a direct circular convolution on small grids; the whole pipeline
 against the reference fixtures (tests/golden/f, R) at the reference's
 tolerances and against the analytic ellipse Radon transform.
 """
+import os
+
 import numpy as np
+import scipy.fft
+
+WORKERS = len(os.sched_getaffinity(0))
 
 POLE = np.sqrt(3.0) - 2.0  # main_kernels.cuh:2
 HORIZON = 12               # main_kernels.cuh:99
@@ -159,7 +164,7 @@ def lp_convolve(F, fZ):
     numpy's irfft2 already divides by Nrho*Ntheta, so the explicit scale is
     folded into it."""
     Nrho, Ntheta = F.shape[-2:]
-    return np.fft.irfft2(np.fft.rfft2(F) * fZ, s=(Nrho, Ntheta))
+    return scipy.fft.irfft2(scipy.fft.rfft2(F, workers=WORKERS) * fZ, s=(Nrho, Ntheta), workers=WORKERS)
 
 
 def fwd(tab, f):

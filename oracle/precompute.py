@@ -216,12 +216,12 @@ def create_fwd(P):
     return parsi, parsf, parts
 
 
-def wint(n, t):
-    """initsadj.py:176-203 (MATLAB twin tests/matlab_files/wint.m:1-20).
-
-    Weights w such that sum(w * f(t)) integrates piecewise degree-(n-1)
-    interpolants of f; the j-loop is vectorised (same sums, other order).
-    """
+def wint_core(n, t):
+    """initsadj.py:176-200 without the tail ramp: weights w such that
+    sum(w * g(t)) = integral of t * g(t) over [t_0, t_end] for every
+    polynomial g of degree < n (degree n-1 interpolation on overlapping
+    windows, each sub-interval averaged over the windows covering it).
+    The j-loop is vectorised (same sums, other order)."""
     t = np.asarray(t, dtype=np.float64)
     N = len(t)
     s = np.linspace(1e-40, 1, n)
@@ -241,6 +241,14 @@ def wint(n, t):
     w = np.zeros(N)
     for i in range(n):
         w[i:i + J] += C[:, i]
+    return w
+
+
+def wint(n, t):
+    """initsadj.py:176-203 (MATLAB twin tests/matlab_files/wint.m:1-20): the
+    core weights with the last 40 replaced by a linear ramp (initsadj.py:202)."""
+    w = wint_core(n, t)
+    N = len(w)
     wn = w.copy()
     wn[-40:] = w[-40] / (N - 40) * np.arange(N - 40, N)
     return wn

@@ -1,0 +1,35 @@
+"""The C ABI library loads and exports every entry point of include/lprec_amd.h
+(CPU only: no compute calls)."""
+import ctypes
+import os
+import re
+
+from lprec_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "lprec_amd.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(lp_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.EXPORTS) == names
+
+
+def test_null_handles_are_rejected():
+    lib = _lib.load()
+    assert lib.lp_fwd(None, None, None, 1, None) == _lib.LP_ERR_ARG
+    assert lib.lp_adj(None, None, None, 1, None) == _lib.LP_ERR_ARG
+    assert lib.lp_init_fwd(None, None, None) == _lib.LP_ERR_ARG
+    assert lib.lp_set_tuning(None, 0, 4) == _lib.LP_ERR_ARG
+    lib.lp_destroy(None)
+    sz = (ctypes.c_int64 * 5)()
+    assert lib.lp_pre_sizes(None, sz) == _lib.LP_ERR_ARG
