@@ -33,7 +33,8 @@ def stage_bytes(g):
     fwd = {
         "prep_rows": 4 * N0 * N0 + 4 * N * N,      # read f, write padded x-filtered image
         "col_iir": 8 * N * N,                      # read + write the image
-        "rows_fwd": 4 * N * N + 3 * L,             # image gathered once, S written per span
+        "gather_lp": 4 * N * N + 3 * L,            # image read once, log-polar grid written per span
+        "rows_fwd": 3 * 2 * L,                     # grid read, half spectra written per span
         "cols": 3 * 2 * L,                         # S read + written per span
         "rows_inv": 3 * 2 * L,                     # S read, G written per span
         "polar_out": 3 * L // 2 + 4 * N0 * P0,     # half of each G (|theta| < beta/2) read, R written
@@ -41,7 +42,8 @@ def stage_bytes(g):
     adj = {
         "sino_rows": 4 * N0 * P0 + 4 * N * P,
         "col_iir": 8 * N * P,
-        "rows_fwd": 4 * N * P + 3 * L,
+        "gather_lp": 4 * N * P + 3 * L,
+        "rows_fwd": 3 * 2 * L,
         "cols": 3 * 2 * L,
         "rows_inv": 3 * 2 * L,
         "cart_out": 3 * L + 4 * N0 * N0,

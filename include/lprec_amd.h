@@ -81,7 +81,8 @@ int lp_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
 int lp_info(const lp_handle* h, int64_t* info);
 /* Kernel schedule knobs: key 0 = log-polar rows per row-FFT block (1..16),
  * key 1 = slices per block in the row/column passes (>=1), key 2 = rows per
- * prefilter column chunk (32, 64 or 128).  Returns LP_ERR_ARG for bad values. */
+ * prefilter column chunk (32, 64 or 128), key 3 = slices per block in the
+ * tiled gathers (>=1).  Returns LP_ERR_ARG for bad values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 
@@ -91,12 +92,26 @@ int lp_get_tuning(const lp_handle* h, int key, int* value);
  * lp_profile(h, 1) enables and clears, lp_profile(h, 0) disables.
  * lp_profile_read synchronises the recorded events and returns, for
  * which = 0 (fwd) or 1 (adj), the summed milliseconds of each stage over all
- * calls since enabling in ms[0..5], and the call count in *ncalls.
- * Stages: fwd = prep rows, column prefilter, row pass, column pass, inverse
- * row pass, polar gather; adj = sinogram rows, column prefilter, row pass,
- * column pass, inverse row pass, Cartesian gather. */
+ * calls since enabling in ms[0..6], and the call count in *ncalls.
+ * Stages: fwd = prep rows, column prefilter, image -> log-polar gather, row
+ * FFT, column pass, inverse row FFT, polar gather; adj = sinogram rows, column
+ * prefilter, sinogram -> log-polar gather, row FFT, column pass, inverse row
+ * FFT, Cartesian gather. */
 int lp_profile(lp_handle* h, int enable);
 int lp_profile_read(lp_handle* h, int which, float* ms, int* ncalls);
+
+/* Stage-wise testing (no reference counterpart).  lp_debug_stop(h, k) makes
+ * the next lp_fwd / lp_adj calls return after stage k (1 = input rows,
+ * 2 = column prefilter, 3 = gather into the log-polar grid, 4 = row FFT,
+ * 5 = column pass, 6 = inverse row FFT; 0 = run everything).  lp_debug_copy
+ * copies an internal device buffer to host memory (synchronising the device):
+ * which = 0 prefiltered image [Nslices][N][N], 1 prefiltered sinogram
+ * [Nslices][Nproj][N], 2 log-polar grids [3][ns0][Nrho][Ntheta] of the last
+ * call, 3 half spectra [3][ns0][Ntheta/2][Nrho] (complex) of the last call,
+ * 4 the row-sweep output of the last call (image or sinogram layout).
+ * `bytes` must not exceed the buffer; returns LP_ERR_ARG otherwise. */
+int lp_debug_stop(lp_handle* h, int stage);
+int lp_debug_copy(lp_handle* h, int which, void* dst, size_t bytes);
 
 #ifdef __cplusplus
 }

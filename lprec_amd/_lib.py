@@ -24,7 +24,7 @@ EXPORTS = (
     "lp_pre_create", "lp_pre_sizes", "lp_pre_copy", "lp_pre_destroy", "lp_pre_error",
     "lp_create", "lp_init_fwd", "lp_init_adj", "lp_fwd", "lp_adj", "lp_destroy",
     "lp_last_error", "lp_mem_info", "lp_info", "lp_set_tuning", "lp_get_tuning",
-    "lp_profile", "lp_profile_read",
+    "lp_profile", "lp_profile_read", "lp_debug_stop", "lp_debug_copy",
 )
 
 
@@ -67,6 +67,8 @@ def load():
         "lp_get_tuning": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int)]),
         "lp_profile": (i32, [vp, i32]),
         "lp_profile_read": (i32, [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]),
+        "lp_debug_stop": (i32, [vp, i32]),
+        "lp_debug_copy": (i32, [vp, i32, vp, ctypes.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -160,8 +162,8 @@ class Handle:
         self._check(self.lib.lp_get_tuning(self.h, int(key), ctypes.byref(v)))
         return v.value
 
-    STAGES = {0: ("prep_rows", "col_iir", "rows_fwd", "cols", "rows_inv", "polar_out"),
-              1: ("sino_rows", "col_iir", "rows_fwd", "cols", "rows_inv", "cart_out")}
+    STAGES = {0: ("prep_rows", "col_iir", "gather_lp", "rows_fwd", "cols", "rows_inv", "polar_out"),
+              1: ("sino_rows", "col_iir", "gather_lp", "rows_fwd", "cols", "rows_inv", "cart_out")}
 
     def profile(self, enable):
         """Enable (and clear) or disable per-stage HIP-event timing."""
@@ -169,10 +171,20 @@ class Handle:
 
     def profile_read(self, which):
         """-> (dict stage -> summed ms, number of calls) for which = 0 fwd / 1 adj."""
-        ms = (ctypes.c_float * 6)()
+        ms = (ctypes.c_float * 7)()
         n = ctypes.c_int()
         self._check(self.lib.lp_profile_read(self.h, int(which), ms, ctypes.byref(n)))
         return dict(zip(self.STAGES[which], list(ms))), n.value
+
+    def debug_stop(self, stage):
+        """Return from the next fwd/adj calls after `stage` (0 = full run)."""
+        self._check(self.lib.lp_debug_stop(self.h, int(stage)))
+
+    def debug_copy(self, which, shape, dtype=np.float32):
+        """Copy internal buffer `which` (0 fc, 1 Rc, 2 grids, 3 spectra) into a new array."""
+        out = np.empty(shape, dtype=dtype)
+        self._check(self.lib.lp_debug_copy(self.h, int(which), out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
+        return out
 
     def close(self):
         if self.h:

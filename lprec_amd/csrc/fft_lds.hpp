@@ -134,9 +134,20 @@ __device__ void fft_stage(float2* buf, int ldm, int nlines, int M, int Ns, const
             int b = t / nbf, i = t - b * nbf;
             int k = i % Ns;
             if (Ns > 1) {
+                // w^q from at most three table reads (w, w^2, w^4) and at most
+                // two products, so every twiddle stays within a few ulp
                 int base = k * tstep;
+                float2 w1 = twiddle(tw, base, dir);
+                float2 wq[R];
+                wq[1] = w1;
+                if (R > 2) wq[2] = twiddle(tw, 2 * base, dir);
+                if (R > 3) wq[3] = cmul(w1, wq[2]);
+                if (R > 4) wq[4] = (R >= 8) ? twiddle(tw, 4 * base, dir) : cmul(wq[2], wq[2]);
+                if (R > 5) wq[5] = cmul(w1, wq[4]);
+                if (R > 6) wq[6] = cmul(wq[2], wq[4]);
+                if (R > 7) wq[7] = cmul(wq[3], wq[4]);
 #pragma unroll
-                for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], twiddle(tw, q * base, dir));
+                for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], wq[q]);
             }
             dft_small<R>(v[c], dir);
             float2* dst = buf + b * ldm + (i - k) * R + k;

@@ -3,6 +3,7 @@
 // lpRgpu (src/include/lpRgpu.cuh:6-81, src/cuda/lpRgpu.cu, params.cu,
 // gridStorage.cu) behind the C ABI of include/lprec_amd.h.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -56,7 +57,7 @@ struct lp_handle {
     int N = 0, N0 = 0, Nt = 0, Nr = 0, P = 0, P0 = 0, Ns = 0, cor = 0, os = 1, h = 0;
     bool cubic = true;
     bool has_fwd = false, has_adj = false, has_filter = false;
-    int R = 8, zgroup = 1, colchunk = 64;
+    int R = 8, zgroup = 1, colchunk = 64, gzgroup = 1;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -64,6 +65,7 @@ struct lp_handle {
 
     float* d_erho = nullptr;
     float* d_G = nullptr;
+    float* d_tmp = nullptr;   // row-sweep output feeding the column sweep (aliases G when it fits)
     float2* d_S = nullptr;
     std::map<int, float2*> tw;
     FftPlan plan_h{}, plan_N{}, plan_M{};
@@ -73,6 +75,17 @@ struct lp_handle {
     float2* d_p2lp = nullptr;
     unsigned char* d_spanrow = nullptr;
     float2* d_fzF = nullptr;
+    int2* d_rangeF = nullptr;
+    int2* d_rangeA = nullptr;
+    // LDS-tiled gathers: tile lists and the largest source box of each
+    lpk::GTile* d_tilesF = nullptr;   // image -> LP (per span)
+    lpk::GTile* d_tilesP = nullptr;   // LP -> polar (per span)
+    lpk::GTile* d_tilesA = nullptr;   // sinogram -> LP (per span)
+    lpk::GTile* d_tilesC = nullptr;   // LP -> Cartesian (groups of 3 spans)
+    int ntilesF = 0, ntilesP = 0, ntilesA = 0, ntilesC = 0;
+    int boxF = 0, boxP = 0, boxA = 0, boxC = 0;
+    int box_cap = 12288;  // floats of LDS per gather block
+    int stop_after = 0;   // stage-wise testing: return after this stage (0 = never)
 
     float* d_Rc = nullptr;
     float2* d_cmapA = nullptr;
@@ -83,7 +96,7 @@ struct lp_handle {
     float* d_H2 = nullptr;
 
     // stage timing: one set of kNStage+1 events per profiled call
-    static constexpr int kNStage = 6;
+    static constexpr int kNStage = 7;
     bool prof = false;
     std::vector<std::vector<hipEvent_t>> prof_sets[2];
     size_t prof_used[2] = {0, 0};
@@ -163,6 +176,85 @@ int check_launch(lp_handle* h, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return h->fail(LP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
     return LP_OK;
+}
+
+inline bool valid2(float2 v) { return std::isfinite(v.x) && std::isfinite(v.y); }
+
+// Source box of one output tile: every cubic tap (floor(x)-1 .. floor(x)+2)
+// of every valid point, in unwrapped source coordinates.
+template <class Coord>
+bool tile_box(Coord coord, int span, int r0, int c0, int nr, int nc, int& x0, int& y0, int& bw, int& bh) {
+    long long xmin = LLONG_MAX, xmax = LLONG_MIN, ymin = LLONG_MAX, ymax = LLONG_MIN;
+    for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nc; ++c) {
+            float2 v = coord(span, r0 + r, c0 + c);
+            if (!valid2(v)) continue;
+            long long ix = (long long)std::floor(v.x), iy = (long long)std::floor(v.y);
+            xmin = std::min(xmin, ix - 1); xmax = std::max(xmax, ix + 2);
+            ymin = std::min(ymin, iy - 1); ymax = std::max(ymax, iy + 2);
+        }
+    if (xmin > xmax) { x0 = y0 = bw = bh = 0; return false; }
+    if (xmax - xmin > (1 << 20) || ymax - ymin > (1 << 20)) throw std::runtime_error("coordinate table out of range");
+    x0 = (int)xmin; y0 = (int)ymin; bw = (int)(xmax - xmin + 1); bh = (int)(ymax - ymin + 1);
+    return true;
+}
+
+// Tiles of the output rectangle rows [rb, re) x cols [0, nc_all), split until
+// every span's box fits `cap` floats.  `spans` tiles are emitted per output
+// tile (one per span, consecutive).  keep(r0, c0, nr, nc) may drop a tile.
+template <class Coord, class Keep>
+void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>& spans, int r0, int c0, int nr,
+               int nc, int cap, Keep keep, int& maxbox) {
+    if (!keep(r0, c0, nr, nc)) return;
+    std::vector<lpk::GTile> t(spans.size());
+    bool fits = true;
+    for (size_t i = 0; i < spans.size(); ++i) {
+        lpk::GTile& g = t[i];
+        g.r0 = r0; g.c0 = c0; g.nr = nr; g.nc = nc; g.span = spans[i]; g.pad = 0;
+        tile_box(coord, spans[i], r0, c0, nr, nc, g.x0, g.y0, g.bw, g.bh);
+        if ((long long)g.bw * g.bh > cap) fits = false;
+    }
+    if (!fits) {
+        if (nr == 1 && nc == 1) throw std::runtime_error("a single sample exceeds the gather box budget");
+        if (nr >= nc) {
+            int h = nr / 2;
+            add_tiles(out, coord, spans, r0, c0, h, nc, cap, keep, maxbox);
+            add_tiles(out, coord, spans, r0 + h, c0, nr - h, nc, cap, keep, maxbox);
+        } else {
+            int h = nc / 2;
+            add_tiles(out, coord, spans, r0, c0, nr, h, cap, keep, maxbox);
+            add_tiles(out, coord, spans, r0, c0 + h, nr, nc - h, cap, keep, maxbox);
+        }
+        return;
+    }
+    for (auto& g : t) {
+        maxbox = std::max(maxbox, g.bw * g.bh);
+        out.push_back(g);
+    }
+}
+
+template <class Coord, class Keep>
+std::vector<lpk::GTile> tile_grid(Coord coord, const std::vector<int>& spans, int rb, int re, int ncols, int TR, int TC,
+                                  int cap, Keep keep, int& maxbox) {
+    std::vector<lpk::GTile> out;
+    for (int r = rb; r < re; r += TR)
+        for (int c = 0; c < ncols; c += TC)
+            add_tiles(out, coord, spans, r, c, std::min(TR, re - r), std::min(TC, ncols - c), cap, keep, maxbox);
+    return out;
+}
+
+// [lo, hi) theta bounds of the non-empty points of each (span, row) of a dense map
+std::vector<int2> row_ranges(const std::vector<float2>& cmap, int Nr, int Nt) {
+    std::vector<int2> r(3 * (size_t)Nr);
+    for (size_t k = 0; k < 3; ++k)
+        for (int i = 0; i < Nr; ++i) {
+            const float2* row = cmap.data() + (k * Nr + i) * (size_t)Nt;
+            int lo = Nt, hi = 0;
+            for (int j = 0; j < Nt; ++j)
+                if (row[j].x == row[j].x) { lo = std::min(lo, j); hi = j + 1; }
+            r[k * Nr + i] = make_int2(lo < hi ? lo : 0, lo < hi ? hi : 0);
+        }
+    return r;
 }
 
 // kernel spectrum [Nrho][H] interleaved (wire format) -> transposed [H][Nrho]
@@ -254,6 +346,11 @@ int lp_create(const float* g, int device, lp_handle** out) {
         g_create_err = h->err;
         return rc;
     }
+    {
+        size_t need = (size_t)h->Ns * std::max((size_t)h->N * h->N, (size_t)h->P * h->N);
+        if (need <= 3 * (size_t)h->Ns * npts) h->d_tmp = h->d_G;
+        else if ((rc = h->alloc(&h->d_tmp, need))) { g_create_err = h->err; return rc; }
+    }
     *out = h.release();
     return LP_OK;
 }
@@ -308,9 +405,48 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
             if (v.x != v.x) return h->fail(LP_ERR_ARG, "pids do not cover every kept sinogram point");
         }
     for (auto& s : spanrow) if (s == 255) s = 0;
+    for (auto& v : cmap) if (!valid2(v)) v = make_float2(nanv, nanv);
+    for (auto& v : p2lp) if (!valid2(v)) v = make_float2(nanv, nanv);
     std::vector<float2> fzT = transpose_fz(fz, Nr, Nt);
     int rc;
-    if ((rc = h->upload(&h->d_cmapF, cmap)) || (rc = h->upload(&h->d_p2lp, p2lp)) ||
+    std::vector<int2> rng = row_ranges(cmap, Nr, Nt);
+    std::vector<lpk::GTile> tF, tP;
+    try {
+        // image -> LP gather: tiles of 32 rho x 64 theta per span, kept where
+        // they meet the non-empty theta range of one of their rows
+        for (int k = 0; k < 3; ++k) {
+            auto coord = [&](int, int r, int c) { return cmap[(size_t)k * npts + (size_t)r * Nt + c]; };
+            auto keep = [&](int r0, int c0, int nr, int nc) {
+                for (int r = r0; r < r0 + nr; ++r) {
+                    int2 q = rng[(size_t)k * Nr + r];
+                    if (q.x < c0 + nc && q.y > c0) return true;
+                }
+                return false;
+            };
+            auto t = tile_grid(coord, {k}, 0, Nr, Nt, 32, 64, h->box_cap, keep, h->boxF);
+            tF.insert(tF.end(), t.begin(), t.end());
+        }
+        // LP -> polar gather over the kept angles (user rows), per span
+        const int P0 = h->P0, lb = N / 2 - h->cor, os = h->os, N0 = h->N0;
+        for (int k = 0; k < 3; ++k) {
+            int ab = -1, ae = -1;
+            for (int a = 0; a < P0; ++a)
+                if (spanrow[(size_t)a * os] == k) { if (ab < 0) ab = a; ae = a + 1; }
+            if (ab < 0) continue;
+            for (int a = ab; a < ae; ++a)
+                if (spanrow[(size_t)a * os] != k) return h->fail(LP_ERR_UNSUPPORTED, "span rows are not contiguous");
+            auto coord = [&](int, int a, int x) { return p2lp[(size_t)a * os * N + x + lb]; };
+            auto keep = [](int, int, int, int) { return true; };
+            auto t = tile_grid(coord, {k}, ab, ae, N0, 8, 256, h->box_cap, keep, h->boxP);
+            tP.insert(tP.end(), t.begin(), t.end());
+        }
+    } catch (const std::exception& e) {
+        return h->fail(LP_ERR_UNSUPPORTED, e.what());
+    }
+    h->ntilesF = (int)tF.size();
+    h->ntilesP = (int)tP.size();
+    if ((rc = h->upload(&h->d_tilesF, tF)) || (rc = h->upload(&h->d_tilesP, tP))) return rc;
+    if ((rc = h->upload(&h->d_cmapF, cmap)) || (rc = h->upload(&h->d_rangeF, rng)) || (rc = h->upload(&h->d_p2lp, p2lp)) ||
         (rc = h->upload(&h->d_spanrow, spanrow)) || (rc = h->upload(&h->d_fzF, fzT)) ||
         (rc = h->alloc(&h->d_fc, (size_t)h->Ns * N * N)))
         return rc;
@@ -389,9 +525,37 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         if (id < 0 || (size_t)id >= NN) return h->fail(LP_ERR_ARG, "adj cids index out of range");
         for (int k = 0; k < 3; ++k) cmapC[k * NN + id] = make_float2(C1[k][t], C2[k][t]);
     }
+    for (auto& v : cmap) if (!valid2(v)) v = make_float2(nanv, nanv);
+    for (auto& v : cmapC) if (!valid2(v)) v = make_float2(nanv, nanv);
     std::vector<float2> fzT = transpose_fz(fz, Nr, Nt);
     int rc;
-    if ((rc = h->upload(&h->d_cmapA, cmap)) || (rc = h->upload(&h->d_cmapC, cmapC)) ||
+    std::vector<int2> rng = row_ranges(cmap, Nr, Nt);
+    std::vector<lpk::GTile> tA, tC;
+    try {
+        for (int k = 0; k < 3; ++k) {
+            auto coord = [&](int, int r, int c) { return cmap[(size_t)k * npts + (size_t)r * Nt + c]; };
+            auto keep = [&](int r0, int c0, int nr, int nc) {
+                for (int r = r0; r < r0 + nr; ++r) {
+                    int2 q = rng[(size_t)k * Nr + r];
+                    if (q.x < c0 + nc && q.y > c0) return true;
+                }
+                return false;
+            };
+            auto t = tile_grid(coord, {k}, 0, Nr, Nt, 32, 64, h->box_cap, keep, h->boxA);
+            tA.insert(tA.end(), t.begin(), t.end());
+        }
+        // LP -> Cartesian: user image tiles, the three spans of a tile consecutive
+        const int o = N / 2 - h->N0 / 2;
+        auto coordC = [&](int k, int y, int x) { return cmapC[(size_t)k * NN + (size_t)(y + o) * N + x + o]; };
+        auto keepC = [](int, int, int, int) { return true; };
+        tC = tile_grid(coordC, {0, 1, 2}, 0, h->N0, h->N0, 32, 64, h->box_cap, keepC, h->boxC);
+    } catch (const std::exception& e) {
+        return h->fail(LP_ERR_UNSUPPORTED, e.what());
+    }
+    h->ntilesA = (int)tA.size();
+    h->ntilesC = (int)tC.size();
+    if ((rc = h->upload(&h->d_tilesA, tA)) || (rc = h->upload(&h->d_tilesC, tC))) return rc;
+    if ((rc = h->upload(&h->d_cmapA, cmap)) || (rc = h->upload(&h->d_rangeA, rng)) || (rc = h->upload(&h->d_cmapC, cmapC)) ||
         (rc = h->upload(&h->d_fzA, fzT)) || (rc = h->alloc(&h->d_Rc, (size_t)h->Ns * h->P * N)))
         return rc;
     if (any_extra && ((rc = h->upload(&h->d_xrow, xrow)) || (rc = h->upload(&h->d_xent, xent)))) return rc;
@@ -440,21 +604,45 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     hipEvent_t* ev = h->prof_begin(0);
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     mark(0);
-    lpk::launch_image_rows(f, h->d_fc, ns0, h->N0, N, h->cubic, s);
+    lpk::launch_image_rows(f, h->cubic ? h->d_tmp : h->d_fc, ns0, h->N0, N, h->cubic, s);
     mark(1);
-    if (h->cubic) lpk::launch_col_iir(h->d_fc, ns0, N, N, h->colchunk, s);
+    if (h->stop_after == 1) return check_launch(h, "lp_fwd launch");
+    if (h->cubic) lpk::launch_col_iir(h->d_tmp, h->d_fc, ns0, N, N, h->colchunk, s);
     mark(2);
+    if (h->stop_after == 2) return check_launch(h, "lp_fwd launch");
     lpk::RowPass rp = row_pass(h, ns0);
     lpk::Src src{h->d_fc, N, N, (long long)N * N};
-    lpk::launch_rows_fwd(rp, src, h->d_cmapF, h->d_erho, nullptr, nullptr, h->cubic, s);
+    const long long grid = (long long)h->Nr * h->Nt;
+    {   // image -> log-polar samples x e^rho into F (aliases G)
+        lpk::Gather g{};
+        g.src = h->d_fc; g.src_z = (long long)N * N; g.src_span = 0; g.W = N; g.H = N;
+        g.coord = h->d_cmapF; g.c_span = grid; g.c_row = h->Nt; g.c_ro = 0; g.c_co = 0;
+        g.scale = h->d_erho; g.s_row = h->Nt;
+        g.out = h->d_G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxF;
+        lpk::launch_gather(g, h->d_tilesF, h->ntilesF, s);
+    }
     mark(3);
-    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
+    if (h->stop_after == 3) return check_launch(h, "lp_fwd launch");
+    lpk::launch_rows_fwd(rp, h->d_G, h->d_rangeF, src, nullptr, nullptr, h->cubic, s);
     mark(4);
-    lpk::launch_rows_inv(rp, h->d_G, s);
+    if (h->stop_after == 4) return check_launch(h, "lp_fwd launch");
+    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
     mark(5);
-    lpk::launch_polar_out(h->d_G, R, ns0, h->Nt, h->Nr, N, h->N0, h->P0, h->os, N / 2 - h->cor, h->d_p2lp,
-                          h->d_spanrow, h->cubic, s);
+    if (h->stop_after == 5) return check_launch(h, "lp_fwd launch");
+    lpk::launch_rows_inv(rp, h->d_G, s);
     mark(6);
+    if (h->stop_after == 6) return check_launch(h, "lp_fwd launch");
+    {   // log-polar -> polar for the kept angles, cropped detector
+        lpk::Gather g{};
+        g.src = h->d_G; g.src_z = grid; g.src_span = (long long)ns0 * grid; g.W = h->Nt; g.H = h->Nr;
+        g.coord = h->d_p2lp; g.c_span = 0; g.c_row = h->os * N; g.c_ro = 0; g.c_co = N / 2 - h->cor;
+        g.scale = nullptr; g.s_row = 0;
+        g.out = R; g.o_z = (long long)h->P0 * h->N0; g.o_span = 0; g.o_row = h->N0;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxP;
+        lpk::launch_gather(g, h->d_tilesP, h->ntilesP, s);
+    }
+    mark(7);
     return check_launch(h, "lp_fwd launch");
 }
 
@@ -476,20 +664,46 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     hipEvent_t* ev = h->prof_begin(1);
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     mark(0);
-    lpk::launch_sino_rows(R, h->d_Rc, ns0, sp, h->cubic, s);
+    lpk::launch_sino_rows(R, h->cubic ? h->d_tmp : h->d_Rc, ns0, sp, h->cubic, s);
     mark(1);
-    if (h->cubic) lpk::launch_col_iir(h->d_Rc, ns0, h->P, N, h->colchunk, s);
+    if (h->stop_after == 1) return check_launch(h, "lp_adj launch");
+    if (h->cubic) lpk::launch_col_iir(h->d_tmp, h->d_Rc, ns0, h->P, N, h->colchunk, s);
     mark(2);
+    if (h->stop_after == 2) return check_launch(h, "lp_adj launch");
     lpk::RowPass rp = row_pass(h, ns0);
     lpk::Src src{h->d_Rc, N, h->P, (long long)h->P * N};
-    lpk::launch_rows_fwd(rp, src, h->d_cmapA, nullptr, h->d_xrow, h->d_xent, h->cubic, s);
+    const long long grid = (long long)h->Nr * h->Nt;
+    {   // sinogram -> log-polar samples into F (aliases G)
+        lpk::Gather g{};
+        g.src = h->d_Rc; g.src_z = (long long)h->P * N; g.src_span = 0; g.W = N; g.H = h->P;
+        g.coord = h->d_cmapA; g.c_span = grid; g.c_row = h->Nt; g.c_ro = 0; g.c_co = 0;
+        g.scale = nullptr; g.s_row = 0;
+        g.out = h->d_G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxA;
+        lpk::launch_gather(g, h->d_tilesA, h->ntilesA, s);
+    }
     mark(3);
-    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
+    if (h->stop_after == 3) return check_launch(h, "lp_adj launch");
+    lpk::launch_rows_fwd(rp, h->d_G, h->d_rangeA, src, h->d_xrow, h->d_xent, h->cubic, s);
     mark(4);
-    lpk::launch_rows_inv(rp, h->d_G, s);
+    if (h->stop_after == 4) return check_launch(h, "lp_adj launch");
+    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
     mark(5);
-    lpk::launch_cart_out(h->d_G, f, ns0, h->Nt, h->Nr, N, h->N0, h->d_cmapC, h->cubic, s);
+    if (h->stop_after == 5) return check_launch(h, "lp_adj launch");
+    lpk::launch_rows_inv(rp, h->d_G, s);
     mark(6);
+    if (h->stop_after == 6) return check_launch(h, "lp_adj launch");
+    {   // log-polar -> Cartesian, three spans summed, cropped
+        const int o = N / 2 - h->N0 / 2;
+        lpk::Gather g{};
+        g.src = h->d_G; g.src_z = grid; g.src_span = (long long)ns0 * grid; g.W = h->Nt; g.H = h->Nr;
+        g.coord = h->d_cmapC; g.c_span = (long long)N * N; g.c_row = N; g.c_ro = o; g.c_co = o;
+        g.scale = nullptr; g.s_row = 0;
+        g.out = f; g.o_z = (long long)h->N0 * h->N0; g.o_span = 0; g.o_row = h->N0;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 3; g.cubic = h->cubic; g.box_floats = h->boxC;
+        lpk::launch_gather(g, h->d_tilesC, h->ntilesC, s);
+    }
+    mark(7);
     return check_launch(h, "lp_adj launch");
 }
 
@@ -537,6 +751,10 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             if (value != 32 && value != 64 && value != 128) return h->fail(LP_ERR_ARG, "column chunk must be 32, 64 or 128");
             h->colchunk = value;
             return LP_OK;
+        case 3:
+            if (value < 1) return h->fail(LP_ERR_ARG, "slices per gather block must be >= 1");
+            h->gzgroup = value;
+            return LP_OK;
     }
     return h->fail(LP_ERR_ARG, "unknown tuning key");
 }
@@ -569,12 +787,40 @@ int lp_profile_read(lp_handle* h, int which, float* ms, int* ncalls) {
     return LP_OK;
 }
 
+int lp_debug_stop(lp_handle* h, int stage) {
+    if (!h) return LP_ERR_ARG;
+    if (stage < 0 || stage > 6) return h->fail(LP_ERR_ARG, "stage must be in [0, 6]");
+    h->stop_after = stage;
+    return LP_OK;
+}
+
+int lp_debug_copy(lp_handle* h, int which, void* dst, size_t bytes) {
+    if (!h || !dst) return LP_ERR_ARG;
+    DeviceGuard guard(h->device);
+    const void* src = nullptr;
+    size_t cap = 0;
+    switch (which) {
+        case 0: src = h->d_fc; cap = (size_t)h->Ns * h->N * h->N * 4; break;
+        case 1: src = h->d_Rc; cap = (size_t)h->Ns * h->P * h->N * 4; break;
+        case 2: src = h->d_G; cap = 3 * (size_t)h->Ns * h->Nr * h->Nt * 4; break;
+        case 3: src = h->d_S; cap = 3 * (size_t)h->Ns * h->h * h->Nr * 8; break;
+        case 4: src = h->d_tmp; cap = (size_t)h->Ns * std::max((size_t)h->N * h->N, (size_t)h->P * h->N) * 4; break;
+        default: return h->fail(LP_ERR_ARG, "unknown buffer");
+    }
+    if (!src) return h->fail(LP_ERR_ARG, "buffer not allocated (init_fwd / init_adj)");
+    if (bytes > cap) return h->fail(LP_ERR_ARG, "copy larger than the buffer");
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return h->fail(LP_ERR_DEVICE, "device to host copy failed");
+    return LP_OK;
+}
+
 int lp_get_tuning(const lp_handle* h, int key, int* value) {
     if (!h || !value) return LP_ERR_ARG;
     switch (key) {
         case 0: *value = h->R; return LP_OK;
         case 1: *value = h->zgroup; return LP_OK;
         case 2: *value = h->colchunk; return LP_OK;
+        case 3: *value = h->gzgroup; return LP_OK;
     }
     return LP_ERR_ARG;
 }

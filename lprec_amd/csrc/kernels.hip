@@ -168,15 +168,17 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
 // y-direction prefilter (SamplesToCoefficients2DY after the transposes of
 // prefilter2D, lpRgpu.cu:229-239): one thread per column, T rows per chunk with a
 // kHalo-row halo on each side held in registers.
+// Out of place: chunks read halos that neighbouring chunks write.
 template <int T>
-__global__ void k_col_iir(float* __restrict__ a, int L, int W) {
+__global__ void k_col_iir(const float* __restrict__ in, float* __restrict__ outp, int L, int W) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int r0 = blockIdx.y * T;
     const int z = blockIdx.z;
     if (x >= W) return;
     constexpr int K = kHalo;
     constexpr int NV = T + 2 * K;
-    float* col = a + (size_t)z * L * W + x;
+    const float* col = in + (size_t)z * L * W + x;
+    float* ocol = outp + (size_t)z * L * W + x;
     float v[NV];
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
@@ -215,38 +217,33 @@ __global__ void k_col_iir(float* __restrict__ a, int L, int W) {
 #pragma unroll
     for (int n = K; n < K + T; ++n) {
         int row = r0 - K + n;
-        if (row < L) col[(size_t)row * W] = v[n];
+        if (row < L) ocol[(size_t)row * W] = v[n];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Row pass, forward direction: for R log-polar rows of one span, gather the
-// samples (interp kernel, main_kernels.cuh:64-84) times e^rho (mulexp, :87-94),
-// real FFT along theta, store the half spectra transposed into S.
+// Row pass, forward direction: for R log-polar rows of one span, load the
+// gathered samples F (interp + mulexp of lpRgpu.cu:263-267 / 307-311, done by
+// k_gather), real FFT along theta, store the half spectra transposed into S.
 template <int EPT>
-__global__ void k_rows_fwd(RowPass rp, Src src, const float2* __restrict__ cmap, const float* __restrict__ erho,
+__global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* __restrict__ range, Src src,
                            const int* __restrict__ xrow, const float4* __restrict__ xent, int cubic, int ldm) {
     extern __shared__ float2 smc[];
     const int Nt = rp.Ntheta, h = rp.h, R = rp.R, Nr = rp.Nrho;
     const int row0 = blockIdx.x * R, span = blockIdx.y;
     const int z0 = blockIdx.z * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
-    const float2* cm = cmap + ((size_t)span * Nr + row0) * Nt;
-    const float* er = erho ? erho + (size_t)row0 * Nt : nullptr;
+    const int2* rng = range + (size_t)span * Nr + row0;
     for (int z = z0; z < z1; ++z) {
-        const float* T = src.p + (size_t)z * src.zstride;
+        const float* Fz = F + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
         for (int p = threadIdx.x; p < R * Nt; p += blockDim.x) {
             int r = p / Nt, th = p - r * Nt;
-            float2 c = cm[p];
-            float v = 0.f;
-            if (c.x == c.x) {
-                v = sample(T, src.W, src.H, c.x, c.y, cubic);
-                if (er) v *= er[p];
-            }
-            reinterpret_cast<float*>(smc + r * ldm)[th] = v;
+            int2 lr = rng[r];
+            reinterpret_cast<float*>(smc + r * ldm)[th] = (th >= lr.x && th < lr.y) ? Fz[p] : 0.f;
         }
         if (xrow) {
             __syncthreads();
+            const float* T = src.p + (size_t)z * src.zstride;
             for (int r = 0; r < R; ++r) {
                 int lo = xrow[span * (Nr + 1) + row0 + r], hi = xrow[span * (Nr + 1) + row0 + r + 1];
                 for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
@@ -265,6 +262,114 @@ __global__ void k_rows_fwd(RowPass rp, Src src, const float2* __restrict__ cmap,
             out[(size_t)f * Nr + r] = smc[r * ldm + f];
         }
         __syncthreads();
+    }
+}
+
+// LDS-tiled gather (the interp kernel of main_kernels.cuh:64-84 for all four
+// grids): the block stages the source box of its output tile with coalesced
+// row loads, then evaluates every sample of the tile from LDS.  A block owns
+// `group` tiles that share one output tile (the three spans of the
+// LP -> Cartesian gather, lpRgpu.cu:328, summed in span order).
+__global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile* __restrict__ tiles) {
+    extern __shared__ float box[];
+    constexpr int PPT = kGatherPPT;
+    constexpr int MAXG = 3;
+    const int nt = blockDim.x;
+    const GTile* tb = tiles + (size_t)blockIdx.x * g.group;
+    const GTile t0 = tb[0];
+    const int npts = t0.nr * t0.nc;
+    const int z0 = blockIdx.y * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    float2 cc[MAXG][PPT];
+#pragma unroll
+    for (int gi = 0; gi < MAXG; ++gi) {
+        if (gi >= g.group) break;
+        const GTile t = tb[gi];
+#pragma unroll
+        for (int e = 0; e < PPT; ++e) {
+            int p = threadIdx.x + e * nt;
+            cc[gi][e] = make_float2(__int_as_float(0x7fc00000), 0.f);
+            if (p < npts) {
+                int r = p / t.nc, c = p - r * t.nc;
+                cc[gi][e] = g.coord[(size_t)t.span * g.c_span + (size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + c + g.c_co];
+            }
+        }
+    }
+    for (int z = z0; z < z1; ++z) {
+        float acc[PPT];
+#pragma unroll
+        for (int e = 0; e < PPT; ++e) acc[e] = 0.f;
+#pragma unroll
+        for (int gi = 0; gi < MAXG; ++gi) {
+            if (gi >= g.group) break;
+            const GTile t = tb[gi];
+            if (t.bw > 0) {
+                // flattened box, 8 independent loads per thread in flight;
+                // (i + 0.5) / bw in float is exact enough for i, bw < 2^20
+                const float* S = g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span;
+                const int nbox = t.bw * t.bh;
+                const float inv = 1.0f / (float)t.bw;
+                constexpr int UL = 8;
+                for (int i0 = threadIdx.x; i0 < nbox; i0 += UL * nt) {
+                    float v[UL];
+#pragma unroll
+                    for (int u = 0; u < UL; ++u) {
+                        int i = i0 + u * nt;
+                        v[u] = 0.f;
+                        if (i < nbox) {
+                            int yy = (int)(((float)i + 0.5f) * inv);
+                            int xx = i - yy * t.bw;
+                            v[u] = S[(size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W)];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UL; ++u) {
+                        int i = i0 + u * nt;
+                        if (i < nbox) box[i] = v[u];
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < PPT; ++e) {
+                float2 c = cc[gi][e];
+                if (c.x == c.x) {
+                    float fx = floorf(c.x), fy = floorf(c.y);
+                    float ax = c.x - fx, ay = c.y - fy;
+                    int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
+                    int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
+                    const float* q = box + ly * t.bw + lx;
+                    float v;
+                    if (g.cubic) {
+                        float wx[4], wy[4];
+                        bspline_w(ax, wx);
+                        bspline_w(ay, wy);
+                        v = 0.f;
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const float* qr = q + b * t.bw;
+                            v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
+                        }
+                    } else {
+                        const float* q1 = q + t.bw + 1;
+                        const float* q2 = q1 + t.bw;
+                        v = (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
+                    }
+                    acc[e] += v;
+                }
+            }
+            __syncthreads();
+        }
+        float* O = g.out + (size_t)z * g.o_z + (size_t)t0.span * g.o_span;
+#pragma unroll
+        for (int e = 0; e < PPT; ++e) {
+            int p = threadIdx.x + e * nt;
+            if (p < npts) {
+                int r = p / t0.nc, c = p - r * t0.nc;
+                float v = acc[e];
+                if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
+                O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + c] = v;
+            }
+        }
     }
 }
 
@@ -417,22 +522,28 @@ void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, boo
     LP_DISPATCH_EPT(ept, k_sino_rows, dim3(sp.P, nz), dim3(nt), sino_lds_bytes(sp.N), s, R, Rc, sp, (int)cubic);
 }
 
-void launch_col_iir(float* a, int nz, int L, int W, int chunk, hipStream_t s) {
+void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s) {
     dim3 block(256);
     switch (chunk) {
-        case 32: hipLaunchKernelGGL(k_col_iir<32>, dim3((W + 255) / 256, (L + 31) / 32, nz), block, 0, s, a, L, W); break;
-        case 128: hipLaunchKernelGGL(k_col_iir<128>, dim3((W + 255) / 256, (L + 127) / 128, nz), block, 0, s, a, L, W); break;
-        default: hipLaunchKernelGGL(k_col_iir<64>, dim3((W + 255) / 256, (L + 63) / 64, nz), block, 0, s, a, L, W); break;
+        case 32: hipLaunchKernelGGL(k_col_iir<32>, dim3((W + 255) / 256, (L + 31) / 32, nz), block, 0, s, in, out, L, W); break;
+        case 128: hipLaunchKernelGGL(k_col_iir<128>, dim3((W + 255) / 256, (L + 127) / 128, nz), block, 0, s, in, out, L, W); break;
+        default: hipLaunchKernelGGL(k_col_iir<64>, dim3((W + 255) / 256, (L + 63) / 64, nz), block, 0, s, in, out, L, W); break;
     }
 }
 
-void launch_rows_fwd(const RowPass& rp, const Src& src, const float2* cmap, const float* rowscale, const int* xrow,
+void launch_gather(const Gather& g, const GTile* tiles, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    dim3 grid(ntiles / g.group, (g.nz + g.zgroup - 1) / g.zgroup);
+    hipLaunchKernelGGL(k_gather, grid, dim3(kGatherThreads), (size_t)g.box_floats * sizeof(float), s, g, tiles);
+}
+
+void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const Src& src, const int* xrow,
                      const float4* xent, bool cubic, hipStream_t s) {
     int nt = threads_for((long long)rp.R * rp.h);
     int ept = ept_for((long long)rp.R * rp.h, nt);
     int ldm = rp.ldm;
     dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
-    LP_DISPATCH_EPT(ept, k_rows_fwd, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, src, cmap, rowscale, xrow, xent,
+    LP_DISPATCH_EPT(ept, k_rows_fwd, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, F, range, src, xrow, xent,
                     (int)cubic, ldm);
 }
 

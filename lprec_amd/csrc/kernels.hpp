@@ -31,8 +31,9 @@ struct SinoPrep {
     const float2* tw2N;       // W_{2N}^k, k <= N/2
 };
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s);
-// y-direction prefilter of [nz][L][W] in place (columns), chunked with halo
-void launch_col_iir(float* a, int nz, int L, int W, int chunk, hipStream_t s);
+// y-direction prefilter of [nz][L][W] (columns), chunked with halo, in -> out
+// (out of place: the chunks overlap through their halos)
+void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s);
 
 // ---- log-polar convolution passes ------------------------------------------
 struct RowPass {
@@ -44,13 +45,46 @@ struct RowPass {
     const float2* twN;        // W_Ntheta^k
     float2* S;                // [3][nz][h][Nrho]
 };
-// gather + e^rho + R2C along theta; cmap [3][Nrho][Ntheta] (x = NaN: empty)
-void launch_rows_fwd(const RowPass& rp, const Src& src, const float2* cmap, const float* rowscale,
-                     const int* xrow, const float4* xent, bool cubic, hipStream_t s);
+// R2C along theta of the gathered log-polar grids F [3][nz][Nrho][Ntheta];
+// range [3][Nrho]: [lo, hi) theta bounds of the non-empty points of each row
+// (F is read only inside them, zero elsewhere); xrow/xent: extra samples that
+// share a log-polar point with another one (added on top)
+void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const Src& src, const int* xrow,
+                     const float4* xent, bool cubic, hipStream_t s);
 // C2R along theta -> G [3][nz][Nrho][Ntheta]
 void launch_rows_inv(const RowPass& rp, float* G, hipStream_t s);
 // FFT along rho, x fZ, inverse, in place on S; fzT [h+1][Nrho] (scaled)
 void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s);
+
+// ---- LDS-tiled gathers ---------------------------------------------------------
+// One output tile (nr x nc points at (r0, c0)) and the source box [x0, x0+bw) x
+// [y0, y0+bh) (unwrapped, periodic in the source) that holds every tap of its
+// points for span `span`.  Built on the host from the coordinate tables.
+struct GTile {
+    int r0, c0, nr, nc;
+    int x0, y0, bw, bh;
+    int span, pad;
+};
+// out[z*o_z + span*o_span + r*o_row + c] = (sum over the `group` tiles of a
+// block of) sample(src + z*src_z + span*src_span, coord[span*c_span +
+// (r + c_ro)*c_row + c + c_co]) * scale[r*s_row + c]
+struct Gather {
+    const float* src;
+    long long src_z, src_span;
+    int W, H;
+    const float2* coord;
+    long long c_span;
+    int c_row, c_ro, c_co;
+    const float* scale;
+    int s_row;
+    float* out;
+    long long o_z, o_span;
+    int o_row;
+    int nz, zgroup, group, cubic, box_floats;
+};
+constexpr int kGatherThreads = 256;
+constexpr int kGatherPPT = 8;   // points per thread: tiles hold <= 2048 points
+void launch_gather(const Gather& g, const GTile* tiles, int ntiles, hipStream_t s);
 
 // ---- output gathers ---------------------------------------------------------
 // fwd: G -> user R [nz][P0][N0]; p2lp [P][N] (theta idx, rho idx), span [P]
