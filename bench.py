@@ -101,24 +101,40 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup)")
+    ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(slices shard with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
     from lprec_amd import lpTransform
     from lprec_amd.phantom import phantom_stack
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_ref:
+        # known-size copy: reads and writes copy_ref MiB per launch
+        n = args.copy_ref * (1 << 20) // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.empty(n, device="cuda")
+        for _ in range(max(args.steps, 1)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_ref_MiB": args.copy_ref, "launches": max(args.steps, 1)}))
+        return
     N0, P0, ns = args.N, args.nproj, args.slices
     cor = N0 / 2
     lp = lpTransform(N0, P0, ns, "none", cor, "cubic")
     lp.precompute(1)
     lp.initcmem(1, 0)
     h = lp.clphandle[0]
+    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3}
+    for kv in filter(None, args.tune.split(",")):
+        k, v = kv.split("=")
+        h.set_tuning(knobs[k], int(v))
     g = h.info()
     f_host = phantom_stack(ns, N0, 12, seed=2048)
     f = torch.from_numpy(f_host).cuda()
@@ -199,6 +215,7 @@ def main():
                               "frac": round(b_alg * value / HBM_PEAK, 4)},
         "plain_path": {"value": round(plain, 2), "unit": "slices/s", "batched_over_plain": round(value / plain, 3)},
         "stage_ms_per_step": {k: round(v[0], 4) for k, v in stages.items()},
+        "schedule": {k: h.get_tuning(i) for k, i in knobs.items()},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lp, f_host, args.cpu_budget)

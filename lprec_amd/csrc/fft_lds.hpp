@@ -1,6 +1,6 @@
 // Workgroup-cooperative FP32 FFTs held in LDS (gfx950, wave64).
 //
-// A line of M complex values lives at buf[line * ldm + i].  Each stage is a
+// A line of M complex values lives at buf[lidx(line * ldm + i)].  Each stage is a
 // self-sorting Stockham radix-R pass done in place: every thread first pulls
 // the inputs of its butterflies into registers, the workgroup synchronises,
 // then the butterflies are evaluated and scattered to their output slots.
@@ -17,6 +17,13 @@ struct FftPlan {
     int nst;
     int radix[LP_MAX_STAGES];
 };
+
+// LDS slot of complex element x of an FFT buffer: one pad slot after every 16
+// elements, so the Stockham scatter (stride R) and the transposed row/column
+// exchanges spread over all banks.
+__device__ __forceinline__ int lidx(int x) { return x + (x >> 4); }
+// complex slots needed for n elements
+__host__ __device__ constexpr int lds_slots(int n) { return n + (n >> 4) + 1; }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -120,9 +127,9 @@ __device__ void fft_stage(float2* buf, int ldm, int nlines, int M, int Ns, const
         int t = threadIdx.x + c * nt;
         if (t < total) {
             int b = t / nbf, i = t - b * nbf;
-            const float2* src = buf + b * ldm + i;
+            const int base = b * ldm + i;
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[c][q] = src[q * nbf];
+            for (int q = 0; q < R; ++q) v[c][q] = buf[lidx(base + q * nbf)];
         }
     }
     __syncthreads();
@@ -150,9 +157,9 @@ __device__ void fft_stage(float2* buf, int ldm, int nlines, int M, int Ns, const
                 for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], wq[q]);
             }
             dft_small<R>(v[c], dir);
-            float2* dst = buf + b * ldm + (i - k) * R + k;
+            const int dst = b * ldm + (i - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) dst[q * Ns] = v[c][q];
+            for (int q = 0; q < R; ++q) buf[lidx(dst + q * Ns)] = v[c][q];
         }
     }
     __syncthreads();
@@ -176,12 +183,12 @@ __device__ void fft_stage_generic(float2* buf, int ldm, int nlines, int M, int N
             int p = o / nbf, i = o - p * nbf;
             int k = i % Ns;
             int e = tstep * (k + p * Ns);  // exponent of W_M per input index q
-            const float2* src = buf + b * ldm + i;
+            const int src = b * ldm + i;
             float2 acc = make_float2(0.f, 0.f);
             int ex = 0;
             for (int q = 0; q < R; ++q) {
                 float2 w = twiddle(tw, ex, dir);
-                float2 x = src[q * nbf];
+                float2 x = buf[lidx(src + q * nbf)];
                 acc.x += x.x * w.x - x.y * w.y;
                 acc.y += x.x * w.y + x.y * w.x;
                 ex += e;
@@ -198,7 +205,7 @@ __device__ void fft_stage_generic(float2* buf, int ldm, int nlines, int M, int N
             int b = t / M, o = t - b * M;
             int p = o / nbf, i = o - p * nbf;
             int k = i % Ns;
-            buf[b * ldm + (i - k) * R + k + p * Ns] = out[c];
+            buf[lidx(b * ldm + (i - k) * R + k + p * Ns)] = out[c];
         }
     }
     __syncthreads();
@@ -232,20 +239,20 @@ __device__ inline void r2c_post(float2* buf, int ldm, int nlines, int h, const f
     const int npairs = h / 2 + 1;
     for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
         int b = t / npairs, k = t - b * npairs;
-        float2* L = buf + b * ldm;
+        const int L = b * ldm;
         if (k == 0) {
-            float2 z = L[0];
-            L[0] = make_float2(z.x + z.y, z.x - z.y);
+            float2 z = buf[lidx(L)];
+            buf[lidx(L)] = make_float2(z.x + z.y, z.x - z.y);
         } else {
             int kk = h - k;
-            float2 A = L[k], B = L[kk];
+            float2 A = buf[lidx(L + k)], B = buf[lidx(L + kk)];
             float2 Bc = cconj(B);
             float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
             float2 D = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
             float2 O = make_float2(D.y, -D.x);  // D / i
             float2 WO = cmul(twN[k], O);
-            L[k] = cadd(E, WO);
-            if (kk != k) L[kk] = cconj(csub(E, WO));
+            buf[lidx(L + k)] = cadd(E, WO);
+            if (kk != k) buf[lidx(L + kk)] = cconj(csub(E, WO));
         }
     }
     __syncthreads();
@@ -257,26 +264,70 @@ __device__ inline void c2r_pre(float2* buf, int ldm, int nlines, int h, const fl
     const int npairs = h / 2 + 1;
     for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
         int b = t / npairs, k = t - b * npairs;
-        float2* L = buf + b * ldm;
+        const int L = b * ldm;
         if (k == 0) {
-            float2 p = L[0];  // X0 + i Xh
-            L[0] = make_float2(p.x + p.y, p.x - p.y);
+            float2 p = buf[lidx(L)];  // X0 + i Xh
+            buf[lidx(L)] = make_float2(p.x + p.y, p.x - p.y);
         } else {
             int kk = h - k;
-            float2 P = L[k], Q = L[kk];
+            float2 P = buf[lidx(L + k)], Q = buf[lidx(L + kk)];
             float2 S = cadd(P, cconj(Q));
             float2 D = csub(P, cconj(Q));
             float2 w = twN[k];
             float2 iwc = make_float2(w.y, w.x);  // i * conj(w)
-            L[k] = cadd(S, cmul(iwc, D));
+            buf[lidx(L + k)] = cadd(S, cmul(iwc, D));
             if (kk != k) {
                 // Z'[h-k] = conj(S) + i conj(W^{h-k}) (Q - conj P), conj(W^{h-k}) = -W^k
                 float2 S2 = cconj(S);
                 float2 D2 = cconj(make_float2(-D.x, -D.y));  // Q - conj(P) = -conj(D)
                 float2 imw = make_float2(w.y, -w.x);          // -i * w
-                L[kk] = cadd(S2, cmul(imw, D2));
+                buf[lidx(L + kk)] = cadd(S2, cmul(imw, D2));
             }
         }
     }
     __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident radix-16 DFT (16 = 4 x 4 Cooley-Tukey), natural order in
+// and out.  dir = -1 forward, +1 inverse.
+__device__ __forceinline__ void dft16(float2* v, int dir) {
+    float2 a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        dft_small<4>(t, dir);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+    // internal twiddles W16^(n2 k1), W16 = exp(dir 2 pi i / 16)
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, r2 = 0.70710678118654752440f;
+    const float d = (float)dir;
+    a[1][1] = cmul(a[1][1], make_float2(c1, d * s1));
+    a[1][2] = cmul(a[1][2], make_float2(r2, d * r2));
+    a[1][3] = cmul(a[1][3], make_float2(s1, d * c1));
+    a[2][1] = cmul(a[2][1], make_float2(r2, d * r2));
+    a[2][2] = rot90(a[2][2], dir);
+    a[2][3] = cmul(a[2][3], make_float2(-r2, d * r2));
+    a[3][1] = cmul(a[3][1], make_float2(s1, d * c1));
+    a[3][2] = cmul(a[3][2], make_float2(-r2, d * r2));
+    a[3][3] = cmul(a[3][3], make_float2(-c1, -d * s1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        dft_small<4>(t, dir);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
+// twiddle powers w^q, q = 1..R-1, from table reads of w^(2^j) and products
+template <int R>
+__device__ __forceinline__ void twiddle_pows(const float2* __restrict__ tw, int base, int dir, float2* w) {
+    w[1] = twiddle(tw, base, dir);
+#pragma unroll
+    for (int q = 2; q < R; ++q) {
+        if ((q & (q - 1)) == 0) w[q] = twiddle(tw, q * base, dir);        // power of two: table
+        else w[q] = cmul(w[q & (q - 1)], w[q & -q]);                        // product of two table values
+    }
 }

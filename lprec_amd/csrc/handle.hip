@@ -212,7 +212,7 @@ void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>
         lpk::GTile& g = t[i];
         g.r0 = r0; g.c0 = c0; g.nr = nr; g.nc = nc; g.span = spans[i]; g.pad = 0;
         tile_box(coord, spans[i], r0, c0, nr, nc, g.x0, g.y0, g.bw, g.bh);
-        if ((long long)g.bw * g.bh > cap) fits = false;
+        if ((long long)(g.bw | 1) * g.bh > cap) fits = false;
     }
     if (!fits) {
         if (nr == 1 && nc == 1) throw std::runtime_error("a single sample exceeds the gather box budget");
@@ -228,7 +228,7 @@ void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>
         return;
     }
     for (auto& g : t) {
-        maxbox = std::max(maxbox, g.bw * g.bh);
+        maxbox = std::max(maxbox, (g.bw | 1) * g.bh);
         out.push_back(g);
     }
 }
@@ -271,9 +271,10 @@ std::vector<float2> transpose_fz(const float* src, int Nr, int Nt) {
     return out;
 }
 
-// LDS line stride of the row passes: one complex of padding when it fits
+// LDS line stride of the row passes (bank spreading is done by lidx())
 int row_ldm(const lp_handle* h, int R) {
-    return (size_t)R * (h->h + 1) * sizeof(float2) <= h->lds_limit ? h->h + 1 : h->h;
+    (void)R;
+    return h->h;
 }
 
 int pick_rows(lp_handle* h) {
@@ -281,7 +282,7 @@ int pick_rows(lp_handle* h) {
     // register budget allow, up to the tuned value
     int R = h->R;
     while (R > 1) {
-        size_t lds = (size_t)R * row_ldm(h, R) * sizeof(float2);
+        size_t lds = lpk::rows_lds_bytes(row_ldm(h, R), R);
         long long pts = (long long)R * h->h;
         if (lds <= h->lds_limit && pts <= 32LL * 1024 && h->Nr % R == 0) break;
         R /= 2;
@@ -322,7 +323,7 @@ int lp_create(const float* g, int device, lp_handle** out) {
     if (!guard.ok) { g_create_err = "hipSetDevice failed"; return LP_ERR_DEVICE; }
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
-        h->lds_limit = std::min<size_t>((size_t)lds, 65536);
+        h->lds_limit = (size_t)lds;
     h->R = 8;
     h->R = pick_rows(h.get());
     try {

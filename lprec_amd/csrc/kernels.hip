@@ -10,6 +10,8 @@
 // rho, x kernel spectrum, inverse) and an inverse row pass; the texture
 // gathers (main_kernels.cuh:30-84) become exact 16-tap B-spline sums from HBM
 // with periodic addressing.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace lpk {
@@ -61,6 +63,27 @@ __device__ __forceinline__ float sample(const float* __restrict__ T, int W, int 
     return (1.f - ay) * ((1.f - ax) * r0[c0] + ax * r0[c1]) + ay * ((1.f - ax) * r1[c0] + ax * r1[c1]);
 }
 
+// Global -> LDS staging with 8 independent loads in flight per thread.
+// load(i) returns element i (0 <= i < n), store(i, v) writes it to LDS.
+template <class Load, class Store>
+__device__ __forceinline__ void stage_in(int n, Load load, Store store) {
+    constexpr int U = 8;
+    const int nt = blockDim.x;
+    for (int i0 = threadIdx.x; i0 < n; i0 += U * nt) {
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int i = i0 + u * nt;
+            if (i < n) v[u] = load(i);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int i = i0 + u * nt;
+            if (i < n) store(i, v[u]);
+        }
+    }
+}
+
 // Cubic B-spline prefilter of one line held in LDS (ConvertToInterpolationCoefficients,
 // main_kernels.cuh:97-142).  Each thread owns a chunk and restarts the causal and
 // anticausal recursions kHalo samples outside it (truncation |z|^17 < 2e-10), except
@@ -68,7 +91,9 @@ __device__ __forceinline__ float sample(const float* __restrict__ T, int W, int 
 __device__ void iir_line_lds(float* line, float* tmp, int L) {
     const float z = kPole;
     const int nt = blockDim.x;
-    const int T = (L + nt - 1) / nt;
+    // odd chunk length: lanes start T floats apart, so an odd T keeps the
+    // lanes of a wave on distinct banks
+    const int T = ((L + nt - 1) / nt) | 1;
     const int s = threadIdx.x * T;
     const int e = min(s + T, L);
     if (s < L) {
@@ -140,29 +165,37 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
     }
     const int a = ai / sp.os;
     const float* src = R + ((size_t)z * sp.P0 + a) * sp.N0;
-    float* buf = reinterpret_cast<float*>(smc);
-    float* tmp = buf + 2 * N;
+    float* line = reinterpret_cast<float*>(smc + lds_slots(N));   // N floats
+    float* tmp = line + N;                                        // N floats
     for (int x = threadIdx.x; x < N; x += blockDim.x) {
         int xs = min(max(x - sp.lb, 0), sp.N0 - 1);
-        buf[x] = src[xs];
+        line[x] = src[xs];
     }
     if (sp.H2) {
-        for (int x = N + threadIdx.x; x < 2 * N; x += blockDim.x) buf[x] = 0.f;
+        __syncthreads();
+        // length-2N real signal [line, 0] as N complex samples
+        for (int e = threadIdx.x; e < N; e += blockDim.x)
+            smc[lidx(e)] = 2 * e < N ? make_float2(line[2 * e], line[2 * e + 1]) : make_float2(0.f, 0.f);
         __syncthreads();
         fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, -1);
         r2c_post(smc, N, 1, N, sp.tw2N);
         for (int k = threadIdx.x; k < N; k += blockDim.x) {
-            float2 v = smc[k];
-            if (k == 0) smc[0] = make_float2(v.x * sp.H2[0], v.y * sp.H2[N]);
-            else smc[k] = make_float2(v.x * sp.H2[k], v.y * sp.H2[k]);
+            float2 v = smc[lidx(k)];
+            if (k == 0) smc[lidx(0)] = make_float2(v.x * sp.H2[0], v.y * sp.H2[N]);
+            else smc[lidx(k)] = make_float2(v.x * sp.H2[k], v.y * sp.H2[k]);
         }
         __syncthreads();
         c2r_pre(smc, N, 1, N, sp.tw2N);
         fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, +1);
+        for (int e = threadIdx.x; e < N / 2; e += blockDim.x) {
+            float2 v = smc[lidx(e)];
+            line[2 * e] = v.x;
+            line[2 * e + 1] = v.y;
+        }
     }
     __syncthreads();
-    if (cubic) iir_line_lds(buf, tmp, N);
-    for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = buf[x];
+    if (cubic) iir_line_lds(line, tmp, N);
+    for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = line[x];
 }
 
 // y-direction prefilter (SamplesToCoefficients2DY after the transposes of
@@ -236,11 +269,24 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
     const int2* rng = range + (size_t)span * Nr + row0;
     for (int z = z0; z < z1; ++z) {
         const float* Fz = F + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
-        for (int p = threadIdx.x; p < R * Nt; p += blockDim.x) {
-            int r = p / Nt, th = p - r * Nt;
-            int2 lr = rng[r];
-            reinterpret_cast<float*>(smc + r * ldm)[th] = (th >= lr.x && th < lr.y) ? Fz[p] : 0.f;
-        }
+        stage_in(
+            R * h,
+            [&](int p) {
+                int r = p / h, e = p - r * h;
+                int2 lr = rng[r];
+                int th = 2 * e;
+                float2 v = make_float2(0.f, 0.f);
+                if (th + 1 >= lr.x && th < lr.y) {
+                    v = *reinterpret_cast<const float2*>(Fz + (size_t)r * Nt + th);
+                    if (th < lr.x) v.x = 0.f;
+                    if (th + 1 >= lr.y) v.y = 0.f;
+                }
+                return v;
+            },
+            [&](int p, float2 v) {
+                int r = p / h, e = p - r * h;
+                smc[lidx(r * ldm + e)] = v;
+            });
         if (xrow) {
             __syncthreads();
             const float* T = src.p + (size_t)z * src.zstride;
@@ -249,7 +295,8 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
                 for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
                     float4 q = xent[e];
                     float v = sample(T, src.W, src.H, q.y, q.z, cubic);
-                    atomicAdd(reinterpret_cast<float*>(smc + r * ldm) + (int)q.x, v);
+                    int th = (int)q.x;
+                    atomicAdd(reinterpret_cast<float*>(smc + lidx(r * ldm + (th >> 1))) + (th & 1), v);
                 }
             }
         }
@@ -259,7 +306,7 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
         float2* out = rp.S + ((size_t)span * rp.nz + z) * h * (size_t)Nr + row0;
         for (int q = threadIdx.x; q < R * h; q += blockDim.x) {
             int f = q / R, r = q - f * R;
-            out[(size_t)f * Nr + r] = smc[r * ldm + f];
+            out[(size_t)f * Nr + r] = smc[lidx(r * ldm + f)];
         }
         __syncthreads();
     }
@@ -273,40 +320,23 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile* __restrict__ tiles) {
     extern __shared__ float box[];
     constexpr int PPT = kGatherPPT;
-    constexpr int MAXG = 3;
     const int nt = blockDim.x;
     const GTile* tb = tiles + (size_t)blockIdx.x * g.group;
     const GTile t0 = tb[0];
     const int npts = t0.nr * t0.nc;
     const int z0 = blockIdx.y * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
-    float2 cc[MAXG][PPT];
-#pragma unroll
-    for (int gi = 0; gi < MAXG; ++gi) {
-        if (gi >= g.group) break;
-        const GTile t = tb[gi];
-#pragma unroll
-        for (int e = 0; e < PPT; ++e) {
-            int p = threadIdx.x + e * nt;
-            cc[gi][e] = make_float2(__int_as_float(0x7fc00000), 0.f);
-            if (p < npts) {
-                int r = p / t.nc, c = p - r * t.nc;
-                cc[gi][e] = g.coord[(size_t)t.span * g.c_span + (size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + c + g.c_co];
-            }
-        }
-    }
     for (int z = z0; z < z1; ++z) {
         float acc[PPT];
 #pragma unroll
         for (int e = 0; e < PPT; ++e) acc[e] = 0.f;
-#pragma unroll
-        for (int gi = 0; gi < MAXG; ++gi) {
-            if (gi >= g.group) break;
+        for (int gi = 0; gi < g.group; ++gi) {
             const GTile t = tb[gi];
             if (t.bw > 0) {
                 // flattened box, 8 independent loads per thread in flight;
                 // (i + 0.5) / bw in float is exact enough for i, bw < 2^20
                 const float* S = g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span;
                 const int nbox = t.bw * t.bh;
+                const int bs = t.bw | 1;  // odd LDS row stride
                 const float inv = 1.0f / (float)t.bw;
                 constexpr int UL = 8;
                 for (int i0 = threadIdx.x; i0 < nbox; i0 += UL * nt) {
@@ -324,37 +354,46 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
 #pragma unroll
                     for (int u = 0; u < UL; ++u) {
                         int i = i0 + u * nt;
-                        if (i < nbox) box[i] = v[u];
+                        if (i < nbox) {
+                            int yy = (int)(((float)i + 0.5f) * inv);
+                            box[yy * bs + i - yy * t.bw] = v[u];
+                        }
                     }
                 }
             }
             __syncthreads();
-#pragma unroll
+            const float2* cbase = g.coord + (size_t)t.span * g.c_span;
+#pragma unroll 2
             for (int e = 0; e < PPT; ++e) {
-                float2 c = cc[gi][e];
-                if (c.x == c.x) {
-                    float fx = floorf(c.x), fy = floorf(c.y);
-                    float ax = c.x - fx, ay = c.y - fy;
-                    int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
-                    int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
-                    const float* q = box + ly * t.bw + lx;
-                    float v;
-                    if (g.cubic) {
-                        float wx[4], wy[4];
-                        bspline_w(ax, wx);
-                        bspline_w(ay, wy);
-                        v = 0.f;
+                int p = threadIdx.x + e * nt;
+                if (p < npts) {
+                    int r = p / t.nc, cidx = p - r * t.nc;
+                    float2 c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
+                    if (c.x == c.x) {
+                        float fx = floorf(c.x), fy = floorf(c.y);
+                        float ax = c.x - fx, ay = c.y - fy;
+                        const int bs = t.bw | 1;
+                        int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
+                        int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
+                        const float* q = box + ly * bs + lx;
+                        float v;
+                        if (g.cubic) {
+                            float wx[4], wy[4];
+                            bspline_w(ax, wx);
+                            bspline_w(ay, wy);
+                            v = 0.f;
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const float* qr = q + b * t.bw;
-                            v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
+                            for (int b = 0; b < 4; ++b) {
+                                const float* qr = q + b * bs;
+                                v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
+                            }
+                        } else {
+                            const float* q1 = q + bs + 1;
+                            const float* q2 = q1 + bs;
+                            v = (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
                         }
-                    } else {
-                        const float* q1 = q + t.bw + 1;
-                        const float* q2 = q1 + t.bw;
-                        v = (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
+                        acc[e] += v;
                     }
-                    acc[e] += v;
                 }
             }
             __syncthreads();
@@ -383,17 +422,23 @@ __global__ void k_rows_inv(RowPass rp, float* __restrict__ G, int ldm) {
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     for (int z = z0; z < z1; ++z) {
         const float2* in = rp.S + ((size_t)span * rp.nz + z) * h * (size_t)Nr + row0;
-        for (int q = threadIdx.x; q < R * h; q += blockDim.x) {
-            int f = q / R, r = q - f * R;
-            smc[r * ldm + f] = in[(size_t)f * Nr + r];
-        }
+        stage_in(
+            R * h,
+            [&](int q) {
+                int f = q / R, r = q - f * R;
+                return in[(size_t)f * Nr + r];
+            },
+            [&](int q, float2 v) {
+                int f = q / R, r = q - f * R;
+                smc[lidx(r * ldm + f)] = v;
+            });
         __syncthreads();
         c2r_pre(smc, ldm, R, h, rp.twN);
         fft_lds<EPT>(smc, ldm, R, rp.planh, rp.twh, +1);
         float* out = G + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
-        for (int p = threadIdx.x; p < R * Nt; p += blockDim.x) {
-            int r = p / Nt, th = p - r * Nt;
-            out[p] = reinterpret_cast<const float*>(smc + r * ldm)[th];
+        for (int p = threadIdx.x; p < R * h; p += blockDim.x) {
+            int r = p / h, e = p - r * h;
+            *reinterpret_cast<float2*>(out + (size_t)r * Nt + 2 * e) = smc[lidx(r * ldm + e)];
         }
         __syncthreads();
     }
@@ -415,27 +460,154 @@ __global__ void k_cols(RowPass rp, FftPlan planM, const float2* __restrict__ twM
     const float2* fzh = fzT + (size_t)h * M;
     for (int z = z0; z < z1; ++z) {
         float2* col = rp.S + (((size_t)span * rp.nz + z) * h + f) * (size_t)M;
-        for (int i = threadIdx.x; i < M; i += blockDim.x) smc[i] = col[i];
+        stage_in(M, [&](int i) { return col[i]; }, [&](int i, float2 v) { smc[lidx(i)] = v; });
         __syncthreads();
         fft_lds<EPT>(smc, M, 1, planM, twM, -1);
         if (f == 0) {
             for (int k = threadIdx.x; k <= M / 2; k += blockDim.x) {
                 int km = (M - k) % M;
-                float2 A = smc[k], Bc = cconj(smc[km]);
+                float2 A = smc[lidx(k)], Bc = cconj(smc[lidx(km)]);
                 float2 D = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
                 float2 E = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
                 float2 Q = make_float2(E.y, -E.x);  // E / i
                 float2 yk = cadd(cmul(D, fz[k]), rot90(cmul(Q, fzh[k]), 1));
                 float2 ykm = cadd(cmul(cconj(D), fz[km]), rot90(cmul(cconj(Q), fzh[km]), 1));
-                smc[k] = yk;
-                if (km != k) smc[km] = ykm;
+                smc[lidx(k)] = yk;
+                if (km != k) smc[lidx(km)] = ykm;
             }
         } else {
-            for (int i = threadIdx.x; i < M; i += blockDim.x) smc[i] = cmul(smc[i], fz[i]);
+            for (int i = threadIdx.x; i < M; i += blockDim.x) smc[lidx(i)] = cmul(smc[lidx(i)], fz[i]);
         }
         __syncthreads();
         fft_lds<EPT>(smc, M, 1, planM, twM, +1);
-        for (int i = threadIdx.x; i < M; i += blockDim.x) col[i] = smc[i];
+        for (int i = threadIdx.x; i < M; i += blockDim.x) col[i] = smc[lidx(i)];
+        __syncthreads();
+    }
+}
+
+// Column pass for power-of-two M = 16 * MID * 16 (MID in 1, 2, 4, 8, 16):
+// M/16 threads per column hold one radix-16 butterfly each.  Stage 0 reads
+// HBM directly, the last forward stage leaves thread i holding outputs
+// i + q M/16 -- exactly the inputs of the first inverse stage -- so the
+// kernel-spectrum multiply and the inverse start in registers; 4 LDS
+// exchanges in total.  C columns share a 256-thread block for small M.
+template <int MID>
+__device__ __forceinline__ void col_mid_stage(float2* buf, int M, int dir, const float2* __restrict__ tw, int li,
+                                              bool worker) {
+    // radix-MID stage, Ns = 16; thread li handles 16/MID butterflies
+    constexpr int NB = 16 / MID;
+    const int nbf = M / MID;
+    float2 v[NB][MID];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int j = li * NB + b;
+#pragma unroll
+        for (int q = 0; q < MID; ++q) v[b][q] = buf[lidx(j + q * nbf)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int j = li * NB + b;
+        int k = j & 15;
+        const int tstep = M / (16 * MID);
+        if (MID > 1) {
+            float2 w[MID > 1 ? MID : 2];
+            twiddle_pows<MID>(tw, k * tstep, dir, w);
+#pragma unroll
+            for (int q = 1; q < MID; ++q) v[b][q] = cmul(v[b][q], w[q]);
+        }
+        if (MID == 16) dft16(v[b], dir);
+        else if (MID == 8) dft_small<8>(v[b], dir);
+        else if (MID == 4) dft_small<4>(v[b], dir);
+        else if (MID == 2) dft_small<2>(v[b], dir);
+        int dst = (j - k) * MID + k;
+        if (worker) {
+#pragma unroll
+            for (int q = 0; q < MID; ++q) buf[lidx(dst + q * 16)] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// w^q (q = 1..15) applied to v[q] from table reads of w, w^2, w^4, w^8 and at
+// most three products each
+__device__ __forceinline__ void twiddle16(float2* v, const float2* __restrict__ tw, int base, int dir) {
+    float2 w1 = twiddle(tw, base, dir), w2 = twiddle(tw, 2 * base, dir);
+    float2 w4 = twiddle(tw, 4 * base, dir), w8 = twiddle(tw, 8 * base, dir);
+    float2 w3 = cmul(w1, w2);
+    v[1] = cmul(v[1], w1);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], cmul(w1, w4));
+    v[6] = cmul(v[6], cmul(w2, w4));
+    v[7] = cmul(v[7], cmul(w3, w4));
+    v[8] = cmul(v[8], w8);
+    v[9] = cmul(v[9], cmul(w1, w8));
+    v[10] = cmul(v[10], cmul(w2, w8));
+    v[11] = cmul(v[11], cmul(w3, w8));
+    float2 w12 = cmul(w4, w8);
+    v[12] = cmul(v[12], w12);
+    v[13] = cmul(v[13], cmul(w1, w12));
+    v[14] = cmul(v[14], cmul(w2, w12));
+    v[15] = cmul(v[15], cmul(w3, w12));
+}
+
+template <int MID>
+__global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __restrict__ twM,
+                                                   const float2* __restrict__ fzT) {
+    extern __shared__ float2 smc[];
+    // one column per block; block-uniform base pointers with 32-bit lane
+    // offsets keep the 16 addresses out of the vector registers
+    constexpr int M = 256 * MID;            // == rp.Nrho (checked on launch)
+    constexpr int T16 = M / 16;             // working threads per column
+    const int h = rp.h;
+    const int li = min((int)threadIdx.x, T16 - 1);
+    const bool worker = (int)threadIdx.x < T16;
+    const int f = 1 + blockIdx.x, span = blockIdx.y;   // column 0 runs on k_cols
+    float2* buf = smc;
+    const float2* fz = fzT + (size_t)f * M;
+    const int z0 = blockIdx.z * rp.zgroup;
+    const int z1 = min(z0 + rp.zgroup, rp.nz);
+    constexpr int tstep3 = M / (16 * MID * 16);  // last stage: Ns = 16 MID, radix 16 (= 1)
+    for (int z = z0; z < z1; ++z) {
+        float2* col = rp.S + (((size_t)span * rp.nz + z) * h + f) * (size_t)M;
+        float2 v[16];
+        // ---- forward stage 0: radix 16, Ns = 1, straight from HBM
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = col[li + q * T16];
+        dft16(v, -1);
+        if (worker) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) buf[lidx(li * 16 + q)] = v[q];
+        }
+        __syncthreads();
+        if (MID > 1) col_mid_stage<MID>(buf, M, -1, twM, li, worker);
+        // ---- forward last stage: radix 16, Ns = M/16, k = li
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = buf[lidx(li + q * T16)];
+        twiddle16(v, twM, li * tstep3, -1);
+        dft16(v, -1);
+        // v[q] = spectrum at kappa = li + q M/16: kernel multiply in registers
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], fz[li + q * T16]);
+        // ---- inverse stage 0 (radix 16, Ns = 1)
+        dft16(v, +1);
+        __syncthreads();
+        if (worker) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) buf[lidx(li * 16 + q)] = v[q];
+        }
+        __syncthreads();
+        if (MID > 1) col_mid_stage<MID>(buf, M, +1, twM, li, worker);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = buf[lidx(li + q * T16)];
+        twiddle16(v, twM, li * tstep3, +1);
+        dft16(v, +1);
+        if (worker) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) col[li + q * T16] = v[q];
+        }
         __syncthreads();
     }
 }
@@ -490,23 +662,33 @@ int ept_for(long long points, int nt) {
     return -1;
 }
 
-size_t rows_lds_bytes(int ldm, int R) { return (size_t)R * ldm * sizeof(float2); }
-size_t cols_lds_bytes(int M) { return (size_t)M * sizeof(float2); }
-size_t sino_lds_bytes(int N) { return (size_t)(3 * N) * sizeof(float); }
+size_t rows_lds_bytes(int ldm, int R) { return (size_t)lds_slots(R * ldm) * sizeof(float2); }
+size_t cols_lds_bytes(int M) { return (size_t)lds_slots(M) * sizeof(float2); }
+size_t sino_lds_bytes(int N) { return (size_t)lds_slots(N) * sizeof(float2) + (size_t)(2 * N) * sizeof(float); }
+
+// dynamic LDS above 64 KiB (gfx950 has 160 KiB per workgroup) is opted into
+// per kernel; a refusal surfaces as a launch error in check_launch
+template <class K>
+static void allow_lds(K k, size_t bytes) {
+    if (bytes > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
 
 #define LP_DISPATCH_EPT(ept, KER, grid, block, lds, stream, ...)                                   \
     do {                                                                                          \
         switch (ept) {                                                                            \
-            case 4: hipLaunchKernelGGL(KER<4>, grid, block, lds, stream, __VA_ARGS__); break;     \
-            case 8: hipLaunchKernelGGL(KER<8>, grid, block, lds, stream, __VA_ARGS__); break;     \
-            case 16: hipLaunchKernelGGL(KER<16>, grid, block, lds, stream, __VA_ARGS__); break;   \
-            default: hipLaunchKernelGGL(KER<32>, grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 4: allow_lds(KER<4>, lds); hipLaunchKernelGGL(KER<4>, grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 8: allow_lds(KER<8>, lds); hipLaunchKernelGGL(KER<8>, grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 16: allow_lds(KER<16>, lds); hipLaunchKernelGGL(KER<16>, grid, block, lds, stream, __VA_ARGS__); break; \
+            default: allow_lds(KER<32>, lds); hipLaunchKernelGGL(KER<32>, grid, block, lds, stream, __VA_ARGS__); break; \
         }                                                                                         \
     } while (0)
 
+// threads per FFT block: keep <= 16 points per thread (EPT 32 spills past
+// the 128-VGPR budget of a 1024-thread-capable kernel)
 static int threads_for(long long points) {
     int nt = 256;
-    while (points > 32LL * nt && nt < 1024) nt *= 2;
+    while (points > 16LL * nt && nt < 1024) nt *= 2;
     return nt;
 }
 
@@ -555,7 +737,36 @@ void launch_rows_inv(const RowPass& rp, float* G, hipStream_t s) {
     LP_DISPATCH_EPT(ept, k_rows_inv, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, G, ldm);
 }
 
+// middle radix of M = 16 * MID * 16, or 0 when the register kernel does not apply
+static int cols16_mid(int M) {
+    if (M < 256 || M > 4096 || (M & (M - 1))) return 0;
+    return M / 256;
+}
+
 void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s) {
+    if (int mid = cols16_mid(rp.Nrho)) {
+        const int T16 = rp.Nrho / 16;
+        // column 0 (packed theta-DC / Nyquist pair) on the generic kernel
+        {
+            int nt = threads_for(rp.Nrho);
+            int ept = ept_for(rp.Nrho, nt);
+            dim3 g0(1, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+            LP_DISPATCH_EPT(ept, k_cols, g0, dim3(nt), cols_lds_bytes(rp.Nrho), s, rp, planM, twM, fzT);
+        }
+        dim3 grid(rp.h - 1, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+        dim3 block(std::max(T16, 64));
+        size_t lds = (size_t)lds_slots(rp.Nrho) * sizeof(float2);
+        if (rp.h > 1) {
+            switch (mid) {
+                case 1: hipLaunchKernelGGL(k_cols16<1>, grid, block, lds, s, rp, twM, fzT); break;
+                case 2: hipLaunchKernelGGL(k_cols16<2>, grid, block, lds, s, rp, twM, fzT); break;
+                case 4: hipLaunchKernelGGL(k_cols16<4>, grid, block, lds, s, rp, twM, fzT); break;
+                case 8: hipLaunchKernelGGL(k_cols16<8>, grid, block, lds, s, rp, twM, fzT); break;
+                default: hipLaunchKernelGGL(k_cols16<16>, grid, block, lds, s, rp, twM, fzT); break;
+            }
+        }
+        return;
+    }
     int nt = threads_for(rp.Nrho);
     int ept = ept_for(rp.Nrho, nt);
     dim3 grid(rp.h, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
