@@ -612,6 +612,151 @@ __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __r
     }
 }
 
+// Row passes for power-of-two h = 16 * MID * 16: T16 = h/16 threads per row,
+// R rows per block, one radix-16 butterfly per thread per stage.
+template <int MID>
+__device__ __forceinline__ void row_mid_stage(float2* rowbuf, int dir, const float2* __restrict__ tw, int li) {
+    // radix-MID stage (Ns = 16) of one row held at rowbuf (lidx-padded)
+    constexpr int H = 256 * MID;
+    constexpr int NB = 16 / MID;
+    constexpr int nbf = H / MID;
+    float2 v[NB][MID];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int j = li * NB + b;
+#pragma unroll
+        for (int q = 0; q < MID; ++q) v[b][q] = rowbuf[lidx(j + q * nbf)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int j = li * NB + b;
+        int k = j & 15;
+        if (MID > 1) {
+            float2 w[MID > 1 ? MID : 2];
+            twiddle_pows<MID>(tw, k * (H / (16 * MID)), dir, w);
+#pragma unroll
+            for (int q = 1; q < MID; ++q) v[b][q] = cmul(v[b][q], w[q]);
+        }
+        if (MID == 16) dft16(v[b], dir);
+        else if (MID == 8) dft_small<8>(v[b], dir);
+        else if (MID == 4) dft_small<4>(v[b], dir);
+        else if (MID == 2) dft_small<2>(v[b], dir);
+        int dst = (j - k) * MID + k;
+#pragma unroll
+        for (int q = 0; q < MID; ++q) rowbuf[lidx(dst + q * 16)] = v[b][q];
+    }
+    __syncthreads();
+}
+
+template <int MID>
+__global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __restrict__ F, const int2* __restrict__ range,
+                                                     Src src, const int* __restrict__ xrow,
+                                                     const float4* __restrict__ xent, int cubic) {
+    extern __shared__ float2 smc[];
+    constexpr int H = 256 * MID, T16 = 16 * MID, RS = H + H / 16;   // row stride in LDS slots
+    const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
+    const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
+    const int row0 = blockIdx.x * R, span = blockIdx.y;
+    const int z0 = blockIdx.z * rp.zgroup;
+    const int z1 = min(z0 + rp.zgroup, rp.nz);
+    const int2 lr = range[(size_t)span * Nr + row0 + r];
+    float2* rowbuf = smc + r * RS;
+    for (int z = z0; z < z1; ++z) {
+        const float* Fr = F + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int th = 2 * (li + q * T16);
+            float2 x = make_float2(0.f, 0.f);
+            if (th + 1 >= lr.x && th < lr.y) {
+                x = *reinterpret_cast<const float2*>(Fr + th);
+                if (th < lr.x) x.x = 0.f;
+                if (th + 1 >= lr.y) x.y = 0.f;
+            }
+            v[q] = x;
+        }
+        if (xrow) {
+            // rare extra samples sharing a point: add through LDS before the FFT
+#pragma unroll
+            for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
+            __syncthreads();
+            const float* T = src.p + (size_t)z * src.zstride;
+            for (int rr = 0; rr < R; ++rr) {
+                int lo = xrow[span * (Nr + 1) + row0 + rr], hi = xrow[span * (Nr + 1) + row0 + rr + 1];
+                for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+                    float4 qd = xent[e];
+                    float sv = sample(T, src.W, src.H, qd.y, qd.z, cubic);
+                    int th = (int)qd.x;
+                    atomicAdd(reinterpret_cast<float*>(smc + rr * RS + lidx(th >> 1)) + (th & 1), sv);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
+            __syncthreads();
+        }
+        dft16(v, -1);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rowbuf[lidx(li * 16 + q)] = v[q];
+        __syncthreads();
+        if (MID > 1) row_mid_stage<MID>(rowbuf, -1, rp.twh, li);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
+        twiddle16(v, rp.twh, li, -1);
+        dft16(v, -1);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
+        __syncthreads();
+        // lidx(r * H + k) == r * RS + lidx(k): the shared helpers see the same layout
+        r2c_post(smc, H, R, H, rp.twN);
+        float2* out = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
+        for (int qq = threadIdx.x; qq < R * H; qq += blockDim.x) {
+            int f = qq / R, rr = qq - f * R;
+            out[(size_t)f * Nr + rr] = smc[rr * RS + lidx(f)];
+        }
+        __syncthreads();
+    }
+}
+
+template <int MID>
+__global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restrict__ G) {
+    extern __shared__ float2 smc[];
+    constexpr int H = 256 * MID, T16 = 16 * MID, RS = H + H / 16;
+    const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
+    const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
+    const int row0 = blockIdx.x * R, span = blockIdx.y;
+    const int z0 = blockIdx.z * rp.zgroup;
+    const int z1 = min(z0 + rp.zgroup, rp.nz);
+    float2* rowbuf = smc + r * RS;
+    for (int z = z0; z < z1; ++z) {
+        const float2* in = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
+        stage_in(
+            R * H, [&](int qq) { int f = qq / R, rr = qq - f * R; return in[(size_t)f * Nr + rr]; },
+            [&](int qq, float2 v) { int f = qq / R, rr = qq - f * R; smc[rr * RS + lidx(f)] = v; });
+        __syncthreads();
+        c2r_pre(smc, H, R, H, rp.twN);
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
+        dft16(v, +1);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rowbuf[lidx(li * 16 + q)] = v[q];
+        __syncthreads();
+        if (MID > 1) row_mid_stage<MID>(rowbuf, +1, rp.twh, li);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
+        twiddle16(v, rp.twh, li, +1);
+        dft16(v, +1);
+        float* out = G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) *reinterpret_cast<float2*>(out + 2 * (li + q * T16)) = v[q];
+        __syncthreads();
+    }
+}
+
 // fwd output: interp LP -> polar (lpRgpu.cu:284) for the kept angles only, fused
 // with copy3Dstep (:191) and the detector crop copy3Dshifted (:192).
 __global__ void k_polar_out(const float* __restrict__ G, float* __restrict__ R, int nz, int Nt, int Nr, int N,
@@ -719,8 +864,34 @@ void launch_gather(const Gather& g, const GTile* tiles, int ntiles, hipStream_t 
     hipLaunchKernelGGL(k_gather, grid, dim3(kGatherThreads), (size_t)g.box_floats * sizeof(float), s, g, tiles);
 }
 
+// middle radix of h = 16 * MID * 16, or 0 when the register kernels do not apply
+static int rows16_mid(const RowPass& rp) {
+    int h = rp.h;
+    if (h < 256 || h > 4096 || (h & (h - 1))) return 0;
+    if ((long long)rp.R * (h / 16) > 1024) return 0;
+    return h / 256;
+}
+static size_t rows16_lds(const RowPass& rp) { return (size_t)rp.R * (rp.h + rp.h / 16) * sizeof(float2) + 64; }
+
+#define LP_DISPATCH_MID(mid, KER, grid, block, lds, stream, ...)                                                  \
+    do {                                                                                                         \
+        switch (mid) {                                                                                           \
+            case 1: allow_lds(KER<1>, lds); hipLaunchKernelGGL(KER<1>, grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 2: allow_lds(KER<2>, lds); hipLaunchKernelGGL(KER<2>, grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 4: allow_lds(KER<4>, lds); hipLaunchKernelGGL(KER<4>, grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 8: allow_lds(KER<8>, lds); hipLaunchKernelGGL(KER<8>, grid, block, lds, stream, __VA_ARGS__); break;   \
+            default: allow_lds(KER<16>, lds); hipLaunchKernelGGL(KER<16>, grid, block, lds, stream, __VA_ARGS__); break; \
+        }                                                                                                        \
+    } while (0)
+
 void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const Src& src, const int* xrow,
                      const float4* xent, bool cubic, hipStream_t s) {
+    if (int mid = rows16_mid(rp)) {
+        dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+        dim3 block(rp.R * (rp.h / 16));
+        LP_DISPATCH_MID(mid, k_rows16_fwd, grid, block, rows16_lds(rp), s, rp, F, range, src, xrow, xent, (int)cubic);
+        return;
+    }
     int nt = threads_for((long long)rp.R * rp.h);
     int ept = ept_for((long long)rp.R * rp.h, nt);
     int ldm = rp.ldm;
@@ -730,6 +901,12 @@ void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const
 }
 
 void launch_rows_inv(const RowPass& rp, float* G, hipStream_t s) {
+    if (int mid = rows16_mid(rp)) {
+        dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+        dim3 block(rp.R * (rp.h / 16));
+        LP_DISPATCH_MID(mid, k_rows16_inv, grid, block, rows16_lds(rp), s, rp, G);
+        return;
+    }
     int nt = threads_for((long long)rp.R * rp.h);
     int ept = ept_for((long long)rp.R * rp.h, nt);
     int ldm = rp.ldm;
