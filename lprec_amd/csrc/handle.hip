@@ -621,7 +621,7 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
         g.scale = h->d_erho; g.s_row = h->Nt;
         g.out = h->d_G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxF;
-        lpk::launch_gather(g, h->d_tilesF, h->ntilesF, s);
+        lpk::launch_gather(g, h->d_tilesF, h->ntilesF, 0, s);
     }
     mark(3);
     if (h->stop_after == 3) return check_launch(h, "lp_fwd launch");
@@ -641,7 +641,7 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
         g.scale = nullptr; g.s_row = 0;
         g.out = R; g.o_z = (long long)h->P0 * h->N0; g.o_span = 0; g.o_row = h->N0;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxP;
-        lpk::launch_gather(g, h->d_tilesP, h->ntilesP, s);
+        lpk::launch_gather(g, h->d_tilesP, h->ntilesP, 2, s);
     }
     mark(7);
     return check_launch(h, "lp_fwd launch");
@@ -681,7 +681,7 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
         g.scale = nullptr; g.s_row = 0;
         g.out = h->d_G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxA;
-        lpk::launch_gather(g, h->d_tilesA, h->ntilesA, s);
+        lpk::launch_gather(g, h->d_tilesA, h->ntilesA, 1, s);
     }
     mark(3);
     if (h->stop_after == 3) return check_launch(h, "lp_adj launch");
@@ -702,7 +702,7 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
         g.scale = nullptr; g.s_row = 0;
         g.out = f; g.o_z = (long long)h->N0 * h->N0; g.o_span = 0; g.o_row = h->N0;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 3; g.cubic = h->cubic; g.box_floats = h->boxC;
-        lpk::launch_gather(g, h->d_tilesC, h->ntilesC, s);
+        lpk::launch_gather(g, h->d_tilesC, h->ntilesC, 3, s);
     }
     mark(7);
     return check_launch(h, "lp_adj launch");

@@ -317,6 +317,9 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // row loads, then evaluates every sample of the tile from LDS.  A block owns
 // `group` tiles that share one output tile (the three spans of the
 // LP -> Cartesian gather, lpRgpu.cu:328, summed in span order).
+// KIND only names the instance (0 image->LP, 1 sinogram->LP, 2 LP->polar,
+// 3 LP->Cartesian) so that profiles separate the four gathers.
+template <int KIND>
 __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile* __restrict__ tiles) {
     extern __shared__ float box[];
     constexpr int PPT = kGatherPPT;
@@ -858,10 +861,16 @@ void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk
     }
 }
 
-void launch_gather(const Gather& g, const GTile* tiles, int ntiles, hipStream_t s) {
+void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
     if (ntiles <= 0) return;
     dim3 grid(ntiles / g.group, (g.nz + g.zgroup - 1) / g.zgroup);
-    hipLaunchKernelGGL(k_gather, grid, dim3(kGatherThreads), (size_t)g.box_floats * sizeof(float), s, g, tiles);
+    size_t lds = (size_t)g.box_floats * sizeof(float);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_gather<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 1: hipLaunchKernelGGL(k_gather<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 2: hipLaunchKernelGGL(k_gather<2>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        default: hipLaunchKernelGGL(k_gather<3>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+    }
 }
 
 // middle radix of h = 16 * MID * 16, or 0 when the register kernels do not apply

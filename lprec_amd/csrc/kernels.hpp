@@ -84,7 +84,8 @@ struct Gather {
 };
 constexpr int kGatherThreads = 256;
 constexpr int kGatherPPT = 8;   // points per thread: tiles hold <= 2048 points
-void launch_gather(const Gather& g, const GTile* tiles, int ntiles, hipStream_t s);
+// kind: 0 image->LP, 1 sinogram->LP, 2 LP->polar, 3 LP->Cartesian (names the kernel instance)
+void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hipStream_t s);
 
 // ---- output gathers ---------------------------------------------------------
 // fwd: G -> user R [nz][P0][N0]; p2lp [P][N] (theta idx, rho idx), span [P]
