@@ -59,6 +59,87 @@ def pipeline_bytes(g):
     return (4 * N0 * N0 + 16 * N * N + 12 * L + 4 * N0 * P0) + (4 * N0 * P0 + 16 * N * P + 12 * L + 4 * N0 * N0)
 
 
+# Kernel -> stage of the pipeline (one fwd or adj call launches each stage once).
+_STAGE_OF_KERNEL = (
+    ("k_image_rows", "prep_rows"), ("k_sino_rows", "sino_rows"), ("k_col_iir", "col_iir"),
+    ("k_gather<0>", "gather_lp"), ("k_gather<1>", "gather_lp"), ("k_gather<2>", "polar_out"),
+    ("k_gather<3>", "cart_out"), ("k_rows16_fwd", "rows_fwd"), ("k_rows_fwd", "rows_fwd"),
+    ("k_cols16", "cols"), ("k_cols", "cols"), ("k_rows16_inv", "rows_inv"), ("k_rows_inv", "rows_inv"),
+)
+
+
+def _stage_of(kernel_name):
+    base = kernel_name.replace("void ", "").split("(")[0].replace("lpk::", "")
+    for key, stage in _STAGE_OF_KERNEL:
+        if base == key or (not key.endswith(">") and base.startswith(key + "<")):
+            return stage
+    return None
+
+
+def pmc_stage_kib(csv_path):
+    """Sum a rocprofv3 --pmc counter (KiB per dispatch) per pipeline stage and
+    average over calls: -> {"fwd.<stage>": KiB per launch, "adj.<stage>": ...}.
+    Dispatches are walked in order; a call starts at its first-stage kernel."""
+    import csv
+    import collections
+    rows = sorted(csv.DictReader(open(csv_path)), key=lambda r: int(r["Dispatch_Id"]))
+    tot, calls, d = collections.defaultdict(float), collections.Counter(), None
+    for r in rows:
+        st = _stage_of(r["Kernel_Name"])
+        if st is None:
+            continue
+        if st in ("prep_rows", "sino_rows"):
+            d = "fwd" if st == "prep_rows" else "adj"
+            calls[d] += 1
+        if d is not None:
+            tot[d + "." + st] += float(r["Counter_Value"])
+    return {k: v / calls[k[:3]] for k, v in tot.items()}
+
+
+def pmc_copy_factor(csv_path, mib):
+    """Bytes per counter KiB, from a known-size device copy (mib MiB read and
+    written per launch; the largest-count kernel of the run is the copy)."""
+    import csv
+    vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(csv_path))
+            if "copyBuffer" in r["Kernel_Name"]]
+    return mib * (1 << 20) / (sum(vals) / len(vals))
+
+
+def pmc_summary(root, copy_mib):
+    """Calibrated HBM traffic per stage launch from four rocprofv3 --pmc runs
+    under root: fetch_bench/ and write_bench/ (FETCH_SIZE / WRITE_SIZE over the
+    bench), fetch_copy/ and write_copy/ (the same counters over --copy-ref)."""
+    csvp = lambda n: os.path.join(root, n, "run_counter_collection.csv")
+    ff = pmc_copy_factor(csvp("fetch_copy"), copy_mib)
+    fw = pmc_copy_factor(csvp("write_copy"), copy_mib)
+    fetch = pmc_stage_kib(csvp("fetch_bench"))
+    write = pmc_stage_kib(csvp("write_bench"))
+    return {"fetch_bytes_per_KiB": ff, "write_bytes_per_KiB": fw,
+            "stages": {k: [int(fetch[k] * ff), int(write.get(k, 0.0) * fw)] for k in fetch}}
+
+
+# HBM traffic per launch (bytes read, bytes written) of each stage at the
+# default workload and schedule, from pmc_summary() over the rocprofv3 runs
+# recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
+# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
+MEASURED_TRAFFIC = {
+    "fwd.prep_rows": (536900783, 536870912),
+    "fwd.col_iir": (795737614, 536870912),
+    "fwd.gather_lp": (5527993134, 1133248512),
+    "fwd.rows_fwd": (1411006259, 3422552064),
+    "fwd.cols": (4831650994, 3221323776),
+    "fwd.rows_inv": (6445603614, 3221225728),
+    "fwd.polar_out": (5262708756, 393216000),
+    "adj.sino_rows": (393264505, 786432000),
+    "adj.col_iir": (1168524964, 786432000),
+    "adj.gather_lp": (2593658431, 1610686480),
+    "adj.rows_fwd": (1957611386, 3422552064),
+    "adj.cols": (4831645810, 3221323776),
+    "adj.rows_inv": (6445715164, 3221225664),
+    "adj.cart_out": (5066737552, 536870912),
+}
+
+
 def cpu_baseline(lp, f0, budget_s):
     """Time the FP64 oracle mirror (numpy/scipy) on one slice of the workload."""
     import numpy as np
@@ -104,7 +185,13 @@ def main():
     ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--pmc-summary", default="", metavar="DIR",
+                    help="print calibrated per-stage traffic from rocprofv3 --pmc runs under DIR and exit")
+    ap.add_argument("--pmc-copy-mib", type=int, default=4096)
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary, args.pmc_copy_mib), indent=1))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(slices shard with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -192,6 +279,8 @@ def main():
     dom_ms, dom_bytes = stages[dom]
     achieved = dom_bytes / (dom_ms * 1e-3)
     b_alg = pipeline_bytes(g)
+    default_run = (N0, P0, ns, args.tune) == (2048, 1500, 32, "")
+    traffic = sum(MEASURED_TRAFFIC[dom]) if default_run and dom in MEASURED_TRAFFIC else None
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -210,8 +299,9 @@ def main():
                                % (g["N"], g["os"], g["Nproj"], g["Nrho"], g["Ntheta"]),
                    "N0": N0, "Nproj": P0, "slices_per_batch": ns, "interp": "cubic", "filter": "none"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "alg_bytes_per_launch": int(dom_bytes), "avg_launch_ms": round(dom_ms, 4)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": int(dom_bytes), "avg_launch_ms": round(dom_ms, 4),
+                     "traffic_read_write": list(MEASURED_TRAFFIC[dom]) if traffic else None},
         "pipeline_roofline": {"B_alg_bytes_per_slice": int(b_alg),
                               "achieved_GBps": round(b_alg * value / 1e9, 1),
                               "frac": round(b_alg * value / HBM_PEAK, 4)},
