@@ -67,6 +67,16 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream);
  * (lpRgpu.cu:196-207): R [ns0][Nproj0][N0] -> f [ns0][N0][N0]; applies the FBP
  * filter when the adjoint tables carry one. */
 int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream);
+/* Exact transposes (no reference counterpart: the reference's adj is an
+ * independently derived operator, initsadj.py:136 vs initsfwd.py:86, and is
+ * only approximately the transpose of fwd).  lp_fwd_T computes f = F^T R for
+ * the operator F of lp_fwd (R [ns0][Nproj0][N0] -> f [ns0][N0][N0]);
+ * lp_adj_T computes R = A^T f for the operator A of lp_adj, including its
+ * FBP filter (f [ns0][N0][N0] -> R [ns0][Nproj0][N0]).  They serve gradient
+ * back-propagation and the dot-product test; they overwrite the handle's
+ * scratch like lp_fwd / lp_adj. */
+int lp_fwd_T(lp_handle* h, float* f, const float* R, int ns0, void* stream);
+int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream);
 /* Replaces lpRgpu::~lpRgpu (lpRgpu.cu:41-58).  Safe on NULL. */
 void lp_destroy(lp_handle* h);
 /* Message of the last failed call on this handle. */

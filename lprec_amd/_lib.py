@@ -22,7 +22,7 @@ LP_ERR_UNSUPPORTED = -4
 # every symbol declared in include/lprec_amd.h
 EXPORTS = (
     "lp_pre_create", "lp_pre_sizes", "lp_pre_copy", "lp_pre_destroy", "lp_pre_error",
-    "lp_create", "lp_init_fwd", "lp_init_adj", "lp_fwd", "lp_adj", "lp_destroy",
+    "lp_create", "lp_init_fwd", "lp_init_adj", "lp_fwd", "lp_adj", "lp_fwd_T", "lp_adj_T", "lp_destroy",
     "lp_last_error", "lp_mem_info", "lp_info", "lp_set_tuning", "lp_get_tuning",
     "lp_profile", "lp_profile_read", "lp_debug_stop", "lp_debug_copy",
 )
@@ -59,6 +59,8 @@ def load():
         "lp_init_adj": (i32, [vp, vp, vp]),
         "lp_fwd": (i32, [vp, vp, vp, i32, vp]),
         "lp_adj": (i32, [vp, vp, vp, i32, vp]),
+        "lp_fwd_T": (i32, [vp, vp, vp, i32, vp]),
+        "lp_adj_T": (i32, [vp, vp, vp, i32, vp]),
         "lp_destroy": (None, [vp]),
         "lp_last_error": (ctypes.c_char_p, [vp]),
         "lp_mem_info": (i32, [i32, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
@@ -146,6 +148,14 @@ class Handle:
     def adj(self, f_ptr, R_ptr, ns0, stream):
         self._check(self.lib.lp_adj(self.h, ctypes.c_void_p(f_ptr), ctypes.c_void_p(R_ptr), int(ns0),
                                     ctypes.c_void_p(stream)))
+
+    def fwd_T(self, f_ptr, R_ptr, ns0, stream):
+        self._check(self.lib.lp_fwd_T(self.h, ctypes.c_void_p(f_ptr), ctypes.c_void_p(R_ptr), int(ns0),
+                                      ctypes.c_void_p(stream)))
+
+    def adj_T(self, R_ptr, f_ptr, ns0, stream):
+        self._check(self.lib.lp_adj_T(self.h, ctypes.c_void_p(R_ptr), ctypes.c_void_p(f_ptr), int(ns0),
+                                      ctypes.c_void_p(stream)))
 
     def info(self):
         v = (ctypes.c_int64 * 12)()

@@ -590,6 +590,7 @@ static lpk::RowPass row_pass(lp_handle* h, int nz) {
     rp.twh = h->tw[h->h];
     rp.twN = h->tw[h->N];
     rp.S = h->d_S;
+    rp.conj = 0;
     return rp;
 }
 
@@ -706,6 +707,109 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     }
     mark(7);
     return check_launch(h, "lp_adj launch");
+}
+
+// Exact transpose of lp_fwd: R [ns0][P0][N0] -> f [ns0][N0][N0].  The stages
+// of lp_fwd in reverse order, each replaced by its transpose.
+int lp_fwd_T(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
+    if (!h) return LP_ERR_ARG;
+    if (!h->has_fwd) return h->fail(LP_ERR_ARG, "forward tables not initialised (lp_init_fwd)");
+    if (!R || !f) return h->fail(LP_ERR_ARG, "null data pointer");
+    if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
+    DeviceGuard guard(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    const int N = h->N;
+    const long long grid = (long long)h->Nr * h->Nt;
+    lpk::launch_zero(h->d_G, 3 * (size_t)ns0 * grid, s);
+    {   // transpose of the log-polar -> polar gather
+        lpk::Scatter g{};
+        g.dst = h->d_G; g.dst_z = grid; g.dst_span = (long long)ns0 * grid; g.W = h->Nt; g.H = h->Nr;
+        g.coord = h->d_p2lp; g.c_span = 0; g.c_row = h->os * N; g.c_ro = 0; g.c_co = N / 2 - h->cor;
+        g.scale = nullptr; g.s_row = 0;
+        g.val = R; g.v_z = (long long)h->P0 * h->N0; g.v_span = 0; g.v_row = h->N0;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxP;
+        lpk::launch_scatter(g, h->d_tilesP, h->ntilesP, 2, s);
+    }
+    lpk::RowPass rp = row_pass(h, ns0);
+    rp.conj = 1;
+    lpk::Src src{h->d_fc, N, N, (long long)N * N};
+    lpk::launch_rows_fwd(rp, h->d_G, nullptr, src, nullptr, nullptr, h->cubic, s);
+    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
+    lpk::launch_rows_inv(rp, h->d_G, s);
+    float* acc = h->d_fc;   // never aliases G, which the scatter reads
+    lpk::launch_zero(acc, (size_t)ns0 * N * N, s);
+    {   // transpose of the image -> log-polar gather (x e^rho)
+        lpk::Scatter g{};
+        g.dst = acc; g.dst_z = (long long)N * N; g.dst_span = 0; g.W = N; g.H = N;
+        g.coord = h->d_cmapF; g.c_span = grid; g.c_row = h->Nt; g.c_ro = 0; g.c_co = 0;
+        g.scale = h->d_erho; g.s_row = h->Nt;
+        g.val = h->d_G; g.v_z = grid; g.v_span = (long long)ns0 * grid; g.v_row = h->Nt;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxF;
+        lpk::launch_scatter(g, h->d_tilesF, h->ntilesF, 0, s);
+    }
+    if (h->cubic) {
+        lpk::launch_col_iir_T(h->d_fc, h->d_tmp, ns0, N, N, h->colchunk, s);
+        lpk::launch_image_rows_T(h->d_tmp, f, ns0, h->N0, N, true, s);
+    } else {
+        lpk::launch_image_rows_T(acc, f, ns0, h->N0, N, false, s);
+    }
+    return check_launch(h, "lp_fwd_T launch");
+}
+
+// Exact transpose of lp_adj: f [ns0][N0][N0] -> R [ns0][P0][N0].
+int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
+    if (!h) return LP_ERR_ARG;
+    if (!h->has_adj) return h->fail(LP_ERR_ARG, "adjoint tables not initialised (lp_init_adj)");
+    if (!R || !f) return h->fail(LP_ERR_ARG, "null data pointer");
+    if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
+    DeviceGuard guard(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    const int N = h->N;
+    const long long grid = (long long)h->Nr * h->Nt;
+    lpk::launch_zero(h->d_G, 3 * (size_t)ns0 * grid, s);
+    {   // transpose of the log-polar -> Cartesian gather (three spans per tile)
+        const int o = N / 2 - h->N0 / 2;
+        lpk::Scatter g{};
+        g.dst = h->d_G; g.dst_z = grid; g.dst_span = (long long)ns0 * grid; g.W = h->Nt; g.H = h->Nr;
+        g.coord = h->d_cmapC; g.c_span = (long long)N * N; g.c_row = N; g.c_ro = o; g.c_co = o;
+        g.scale = nullptr; g.s_row = 0;
+        g.val = f; g.v_z = (long long)h->N0 * h->N0; g.v_span = 0; g.v_row = h->N0;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 3; g.cubic = h->cubic; g.box_floats = h->boxC;
+        lpk::launch_scatter(g, h->d_tilesC, h->ntilesC, 3, s);
+    }
+    lpk::RowPass rp = row_pass(h, ns0);
+    rp.conj = 1;
+    lpk::Src src{h->d_Rc, N, h->P, (long long)h->P * N};
+    lpk::launch_rows_fwd(rp, h->d_G, nullptr, src, nullptr, nullptr, h->cubic, s);
+    lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
+    lpk::launch_rows_inv(rp, h->d_G, s);
+    float* acc = h->d_Rc;
+    lpk::launch_zero(acc, (size_t)ns0 * h->P * N, s);
+    {   // transpose of the sinogram -> log-polar gather, plus its extra samples
+        lpk::Scatter g{};
+        g.dst = acc; g.dst_z = (long long)h->P * N; g.dst_span = 0; g.W = N; g.H = h->P;
+        g.coord = h->d_cmapA; g.c_span = grid; g.c_row = h->Nt; g.c_ro = 0; g.c_co = 0;
+        g.scale = nullptr; g.s_row = 0;
+        g.val = h->d_G; g.v_z = grid; g.v_span = (long long)ns0 * grid; g.v_row = h->Nt;
+        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxA;
+        lpk::launch_scatter(g, h->d_tilesA, h->ntilesA, 1, s);
+        lpk::launch_scatter_extra(h->d_G, acc, ns0, h->Nt, h->Nr, N, h->P, h->d_xrow, h->d_xent, h->cubic, s);
+    }
+    lpk::SinoPrep sp;
+    sp.N0 = h->N0; sp.N = N; sp.P0 = h->P0; sp.P = h->P; sp.os = h->os; sp.lb = N / 2 - h->cor;
+    sp.H2 = h->has_filter ? h->d_H2 : nullptr;
+    sp.planN = h->plan_N;
+    sp.twN = h->tw[N];
+    sp.tw2N = h->has_filter ? h->tw[2 * N] : h->tw[N];
+    if (h->cubic) {
+        lpk::launch_col_iir_T(acc, h->d_tmp, ns0, h->P, N, h->colchunk, s);
+        lpk::launch_sino_rows_T(h->d_tmp, R, ns0, sp, true, s);
+    } else {
+        lpk::launch_sino_rows_T(acc, R, ns0, sp, false, s);
+    }
+    return check_launch(h, "lp_adj_T launch");
 }
 
 void lp_destroy(lp_handle* h) { delete h; }

@@ -16,10 +16,27 @@
 
 namespace lpk {
 
+// Logical index of hardware block b of a 1-D launch of n blocks.  The
+// dispatcher deals workgroups to the 8 XCDs round-robin (b -> XCD b % 8); this
+// gives each XCD a contiguous range of logical blocks, so logical neighbours
+// (blocks that share source data) run concurrently behind the same L2.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    constexpr int X = 8;
+    const int q = n / X, r = n - q * X;
+    const int x = b % X, s = b / X;
+    return x * q + min(x, r) + s;
+}
+
 __device__ __forceinline__ int wrapi(int i, int n) {
     if ((unsigned)i < (unsigned)n) return i;
     i %= n;
     return i < 0 ? i + n : i;
+}
+
+// kernel-spectrum entry, conjugated for the transposed convolution
+__device__ __forceinline__ float2 fzval(const float2* __restrict__ fz, int i, int conj) {
+    float2 v = fz[i];
+    return conj ? make_float2(v.x, -v.y) : v;
 }
 
 // bspline_weights (main_kernels.cuh:9-19)
@@ -266,14 +283,14 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
     const int row0 = blockIdx.x * R, span = blockIdx.y;
     const int z0 = blockIdx.z * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
-    const int2* rng = range + (size_t)span * Nr + row0;
+    const int2* rng = range ? range + (size_t)span * Nr + row0 : nullptr;
     for (int z = z0; z < z1; ++z) {
         const float* Fz = F + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
         stage_in(
             R * h,
             [&](int p) {
                 int r = p / h, e = p - r * h;
-                int2 lr = rng[r];
+                int2 lr = rng ? rng[r] : make_int2(0, Nt);
                 int th = 2 * e;
                 float2 v = make_float2(0.f, 0.f);
                 if (th + 1 >= lr.x && th < lr.y) {
@@ -324,10 +341,15 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
     extern __shared__ float box[];
     constexpr int PPT = kGatherPPT;
     const int nt = blockDim.x;
-    const GTile* tb = tiles + (size_t)blockIdx.x * g.group;
+    // logical order: the slice groups of one tile are neighbours (they share
+    // the coordinates), then neighbouring tiles (they share source rows)
+    const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int zg = lb % nzg, tile = lb / nzg;
+    const GTile* tb = tiles + (size_t)tile * g.group;
     const GTile t0 = tb[0];
     const int npts = t0.nr * t0.nc;
-    const int z0 = blockIdx.y * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
     for (int z = z0; z < z1; ++z) {
         float acc[PPT];
 #pragma unroll
@@ -473,13 +495,13 @@ __global__ void k_cols(RowPass rp, FftPlan planM, const float2* __restrict__ twM
                 float2 D = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
                 float2 E = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
                 float2 Q = make_float2(E.y, -E.x);  // E / i
-                float2 yk = cadd(cmul(D, fz[k]), rot90(cmul(Q, fzh[k]), 1));
-                float2 ykm = cadd(cmul(cconj(D), fz[km]), rot90(cmul(cconj(Q), fzh[km]), 1));
+                float2 yk = cadd(cmul(D, fzval(fz, k, rp.conj)), rot90(cmul(Q, fzval(fzh, k, rp.conj)), 1));
+                float2 ykm = cadd(cmul(cconj(D), fzval(fz, km, rp.conj)), rot90(cmul(cconj(Q), fzval(fzh, km, rp.conj)), 1));
                 smc[lidx(k)] = yk;
                 if (km != k) smc[lidx(km)] = ykm;
             }
         } else {
-            for (int i = threadIdx.x; i < M; i += blockDim.x) smc[lidx(i)] = cmul(smc[lidx(i)], fz[i]);
+            for (int i = threadIdx.x; i < M; i += blockDim.x) smc[lidx(i)] = cmul(smc[lidx(i)], fzval(fz, i, rp.conj));
         }
         __syncthreads();
         fft_lds<EPT>(smc, M, 1, planM, twM, +1);
@@ -567,10 +589,15 @@ __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __r
     const int h = rp.h;
     const int li = min((int)threadIdx.x, T16 - 1);
     const bool worker = (int)threadIdx.x < T16;
-    const int f = 1 + blockIdx.x, span = blockIdx.y;   // column 0 runs on k_cols
+    // logical order: slice groups, then spans of one column are neighbours
+    // (they read the same kernel-spectrum column); column 0 runs on k_cols
+    const int nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int zg = lb % nzg, t = lb / nzg;
+    const int span = t % 3, f = 1 + t / 3;
     float2* buf = smc;
     const float2* fz = fzT + (size_t)f * M;
-    const int z0 = blockIdx.z * rp.zgroup;
+    const int z0 = zg * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     constexpr int tstep3 = M / (16 * MID * 16);  // last stage: Ns = 16 MID, radix 16 (= 1)
     for (int z = z0; z < z1; ++z) {
@@ -593,7 +620,7 @@ __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __r
         dft16(v, -1);
         // v[q] = spectrum at kappa = li + q M/16: kernel multiply in registers
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], fz[li + q * T16]);
+        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], fzval(fz, li + q * T16, rp.conj));
         // ---- inverse stage 0 (radix 16, Ns = 1)
         dft16(v, +1);
         __syncthreads();
@@ -652,7 +679,7 @@ __device__ __forceinline__ void row_mid_stage(float2* rowbuf, int dir, const flo
     __syncthreads();
 }
 
-template <int MID>
+template <int MID, bool XTRA>
 __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __restrict__ F, const int2* __restrict__ range,
                                                      Src src, const int* __restrict__ xrow,
                                                      const float4* __restrict__ xent, int cubic) {
@@ -660,10 +687,14 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
     constexpr int H = 256 * MID, T16 = 16 * MID, RS = H + H / 16;   // row stride in LDS slots
     const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
-    const int row0 = blockIdx.x * R, span = blockIdx.y;
-    const int z0 = blockIdx.z * rp.zgroup;
+    // logical order: neighbouring row blocks (which share the 128-byte lines
+    // of the transposed spectra) run behind the same L2
+    const int nrb = Nr / R, nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int row0 = (lb % nrb) * R, span = lb / (nrb * nzg);
+    const int z0 = ((lb / nrb) % nzg) * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
-    const int2 lr = range[(size_t)span * Nr + row0 + r];
+    const int2 lr = range ? range[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
     float2* rowbuf = smc + r * RS;
     for (int z = z0; z < z1; ++z) {
         const float* Fr = F + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
@@ -679,8 +710,9 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
             }
             v[q] = x;
         }
-        if (xrow) {
-            // rare extra samples sharing a point: add through LDS before the FFT
+        if (XTRA) {
+            // rare extra samples sharing a point (adj's rho-wrapped points):
+            // add through LDS before the FFT
 #pragma unroll
             for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
             __syncthreads();
@@ -729,8 +761,12 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
     constexpr int H = 256 * MID, T16 = 16 * MID, RS = H + H / 16;
     const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
-    const int row0 = blockIdx.x * R, span = blockIdx.y;
-    const int z0 = blockIdx.z * rp.zgroup;
+    // logical order: neighbouring row blocks (which share the 128-byte lines
+    // of the transposed spectra) run behind the same L2
+    const int nrb = Nr / R, nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int row0 = (lb % nrb) * R, span = lb / (nrb * nzg);
+    const int z0 = ((lb / nrb) % nzg) * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     float2* rowbuf = smc + r * RS;
     for (int z = z0; z < z1; ++z) {
@@ -757,46 +793,6 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
 #pragma unroll
         for (int q = 0; q < 16; ++q) *reinterpret_cast<float2*>(out + 2 * (li + q * T16)) = v[q];
         __syncthreads();
-    }
-}
-
-// fwd output: interp LP -> polar (lpRgpu.cu:284) for the kept angles only, fused
-// with copy3Dstep (:191) and the detector crop copy3Dshifted (:192).
-__global__ void k_polar_out(const float* __restrict__ G, float* __restrict__ R, int nz, int Nt, int Nr, int N,
-                            int N0, int P0, int os, int lb, const float2* __restrict__ p2lp,
-                            const unsigned char* __restrict__ span, int cubic, int zgroup) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
-    if (x >= N0) return;
-    const int ai = a * os;
-    const int k = span[ai];
-    const float2 c = p2lp[(size_t)ai * N + x + lb];
-    const int z0 = blockIdx.z * zgroup, z1 = min(z0 + zgroup, nz);
-    for (int z = z0; z < z1; ++z) {
-        const float* T = G + ((size_t)k * nz + z) * (size_t)Nr * Nt;
-        R[((size_t)z * P0 + a) * N0 + x] = sample(T, Nt, Nr, c.x, c.y, cubic);
-    }
-}
-
-// adj output: interp LP -> Cartesian accumulated over the spans (lpRgpu.cu:328),
-// fused with the centre crop (:206).
-__global__ void k_cart_out(const float* __restrict__ G, float* __restrict__ f, int nz, int Nt, int Nr, int N, int N0,
-                           const float2* __restrict__ cmapC, int cubic, int zgroup) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= N0) return;
-    const int o = N / 2 - N0 / 2;
-    const size_t p = (size_t)(y + o) * N + x + o;
-    const size_t NN = (size_t)N * N;
-    float2 c[3];
-    for (int k = 0; k < 3; ++k) c[k] = cmapC[k * NN + p];
-    const int z0 = blockIdx.z * zgroup, z1 = min(z0 + zgroup, nz);
-    const size_t grid = (size_t)Nr * Nt;
-    for (int z = z0; z < z1; ++z) {
-        float acc = 0.f;
-        for (int k = 0; k < 3; ++k)
-            if (c[k].x == c[k].x) acc += sample(G + ((size_t)k * nz + z) * grid, Nt, Nr, c[k].x, c[k].y, cubic);
-        f[((size_t)z * N0 + y) * N0 + x] = acc;
     }
 }
 
@@ -863,7 +859,7 @@ void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk
 
 void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
     if (ntiles <= 0) return;
-    dim3 grid(ntiles / g.group, (g.nz + g.zgroup - 1) / g.zgroup);
+    dim3 grid((ntiles / g.group) * ((g.nz + g.zgroup - 1) / g.zgroup));
     size_t lds = (size_t)g.box_floats * sizeof(float);
     switch (kind) {
         case 0: hipLaunchKernelGGL(k_gather<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
@@ -893,12 +889,24 @@ static size_t rows16_lds(const RowPass& rp) { return (size_t)rp.R * (rp.h + rp.h
         }                                                                                                        \
     } while (0)
 
+#define LP_DISPATCH_MID2(mid, KER, B, grid, block, lds, stream, ...)                                                        \
+    do {                                                                                                                 \
+        switch (mid) {                                                                                                   \
+            case 1: allow_lds(KER<1, B>, lds); hipLaunchKernelGGL((KER<1, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 2: allow_lds(KER<2, B>, lds); hipLaunchKernelGGL((KER<2, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 4: allow_lds(KER<4, B>, lds); hipLaunchKernelGGL((KER<4, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 8: allow_lds(KER<8, B>, lds); hipLaunchKernelGGL((KER<8, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
+            default: allow_lds(KER<16, B>, lds); hipLaunchKernelGGL((KER<16, B>), grid, block, lds, stream, __VA_ARGS__); break; \
+        }                                                                                                                \
+    } while (0)
+
 void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const Src& src, const int* xrow,
                      const float4* xent, bool cubic, hipStream_t s) {
     if (int mid = rows16_mid(rp)) {
-        dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+        dim3 grid((rp.Nrho / rp.R) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(rp.R * (rp.h / 16));
-        LP_DISPATCH_MID(mid, k_rows16_fwd, grid, block, rows16_lds(rp), s, rp, F, range, src, xrow, xent, (int)cubic);
+        if (xrow) LP_DISPATCH_MID2(mid, k_rows16_fwd, true, grid, block, rows16_lds(rp), s, rp, F, range, src, xrow, xent, (int)cubic);
+        else LP_DISPATCH_MID2(mid, k_rows16_fwd, false, grid, block, rows16_lds(rp), s, rp, F, range, src, xrow, xent, (int)cubic);
         return;
     }
     int nt = threads_for((long long)rp.R * rp.h);
@@ -911,7 +919,7 @@ void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const
 
 void launch_rows_inv(const RowPass& rp, float* G, hipStream_t s) {
     if (int mid = rows16_mid(rp)) {
-        dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+        dim3 grid((rp.Nrho / rp.R) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(rp.R * (rp.h / 16));
         LP_DISPATCH_MID(mid, k_rows16_inv, grid, block, rows16_lds(rp), s, rp, G);
         return;
@@ -939,7 +947,7 @@ void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, con
             dim3 g0(1, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
             LP_DISPATCH_EPT(ept, k_cols, g0, dim3(nt), cols_lds_bytes(rp.Nrho), s, rp, planM, twM, fzT);
         }
-        dim3 grid(rp.h - 1, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+        dim3 grid((rp.h - 1) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(std::max(T16, 64));
         size_t lds = (size_t)lds_slots(rp.Nrho) * sizeof(float2);
         if (rp.h > 1) {
@@ -959,17 +967,359 @@ void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, con
     LP_DISPATCH_EPT(ept, k_cols, grid, dim3(nt), cols_lds_bytes(rp.Nrho), s, rp, planM, twM, fzT);
 }
 
-void launch_polar_out(const float* G, float* R, int nz, int Ntheta, int Nrho, int N, int N0, int P0, int os, int lb,
-                      const float2* p2lp, const unsigned char* span, bool cubic, hipStream_t s) {
-    dim3 grid((N0 + 255) / 256, P0, nz);
-    hipLaunchKernelGGL(k_polar_out, grid, dim3(256), 0, s, G, R, nz, Ntheta, Nrho, N, N0, P0, os, lb, p2lp, span,
-                       (int)cubic, 1);
+// ---------------------------------------------------------------------------
+// Transposes.  Each kernel below is the exact transpose of a stage above; the
+// two chains fwd^T and adj^T run them in reverse stage order.
+
+__global__ void k_zero(float4* __restrict__ p, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__global__ void k_zero_tail(float* __restrict__ p, size_t b, size_t n) {
+    size_t i = b + threadIdx.x;
+    if (i < n) p[i] = 0.f;
 }
 
-void launch_cart_out(const float* G, float* f, int nz, int Ntheta, int Nrho, int N, int N0, const float2* cmapC,
-                     bool cubic, hipStream_t s) {
-    dim3 grid((N0 + 255) / 256, N0, nz);
-    hipLaunchKernelGGL(k_cart_out, grid, dim3(256), 0, s, G, f, nz, Ntheta, Nrho, N, N0, cmapC, (int)cubic, 1);
+// Transposed cubic B-spline prefilter of one line in LDS: the transpose of
+// iir_line_lds.  The reference's recursion is y = A C x with C the causal
+// sweep (12-term initialisation) and A the anticausal sweep; here
+// u = A^T v is a causal sweep s[m] = v[m] + z s[m-1], u = -z s (z/(z-1) s at
+// the end), and C^T u an anticausal sweep t[m] = u[m] + z t[m+1] giving
+// lam t[m], plus lam z^(m+1) t[0] on m < 12 and lam (1+z) t[0] at m = 0.
+__device__ void iir_line_lds_T(float* line, float* tmp, int L) {
+    const float z = kPole;
+    const float kap = z / (z - 1.0f);
+    const int nt = blockDim.x;
+    const int T = ((L + nt - 1) / nt) | 1;
+    const int s = threadIdx.x * T;
+    const int e = min(s + T, L);
+    if (s < L) {
+        int a = max(0, s - kHalo);   // zero before row 0, so a = 0 is exact
+        float c = line[a];
+        if (a >= s) tmp[a] = (a == L - 1) ? kap * c : -z * c;
+        for (int n = a + 1; n < e; ++n) {
+            c = line[n] + z * c;
+            if (n >= s) tmp[n] = (n == L - 1) ? kap * c : -z * c;
+        }
+    }
+    __syncthreads();
+    if (s < L) {
+        int b = min(e - 1 + kHalo, L - 1);
+        float c = tmp[b];
+        if (b < e) line[b] = kLambda * c;
+        for (int n = b - 1; n >= s; --n) {
+            c = tmp[n] + z * c;
+            if (n < e) line[n] = kLambda * c;
+        }
+    }
+    __syncthreads();
+    const float l0 = line[0];   // lam t[0]
+    __syncthreads();
+    const int hz = min(kHorizon, L);
+    for (int m = threadIdx.x; m < hz; m += nt) {
+        float zp = z;
+        for (int q = 0; q < m; ++q) zp *= z;   // z^(m+1)
+        line[m] = (m == 0) ? (1.0f + z) * l0 : line[m] + zp * l0;
+    }
+    __syncthreads();
+}
+
+// Transpose of k_col_iir: same T-row chunks with a kHalo halo on each side.
+template <int T>
+__global__ void k_col_iir_T(const float* __restrict__ in, float* __restrict__ outp, int L, int W) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r0 = blockIdx.y * T;
+    const int z = blockIdx.z;
+    if (x >= W) return;
+    constexpr int K = kHalo;
+    constexpr int NV = T + 2 * K;
+    const float* col = in + (size_t)z * L * W + x;
+    float* ocol = outp + (size_t)z * L * W + x;
+    const float zp = kPole;
+    const float kap = zp / (zp - 1.0f);
+    float v[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+        int row = r0 - K + n;
+        v[n] = (row >= 0 && row < L) ? col[(size_t)row * W] : 0.f;
+    }
+    // transposed anticausal stage: causal sweep (zeros before row 0 make the
+    // first chunk exact), u = -z s, kap s at row L-1, nothing past it
+    float c = 0.f;
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+        int row = r0 - K + n;
+        c = v[n] + zp * c;
+        v[n] = (row < L - 1) ? -zp * c : (row == L - 1 ? kap * c : 0.f);
+    }
+    // transposed causal stage: anticausal sweep from the top of the window
+    c = 0.f;
+#pragma unroll
+    for (int n = NV - 1; n >= 0; --n) {
+        c = v[n] + zp * c;
+        v[n] = c;
+    }
+    if (r0 == 0) {
+        // the causal initialisation's column: + z^(m+1) t[0] on m < 12, so
+        // that row 0 holds (1 + z) t[0]
+        const float t0 = v[K];
+        const int hz = min(kHorizon, L);
+        float zq = zp;
+#pragma unroll
+        for (int m = 0; m < kHorizon; ++m) {
+            if (m < hz) v[K + m] += zq * t0;
+            zq *= zp;
+        }
+    }
+#pragma unroll
+    for (int n = K; n < K + T; ++n) {
+        int row = r0 - K + n;
+        if (row < L) ocol[(size_t)row * W] = kLambda * v[n];
+    }
+}
+
+// Transpose of k_image_rows: x-direction prefilter transpose, then the crop
+// back to the user's N0 x N0 image (the transpose of the zero padding).
+__global__ void k_image_rows_T(const float* __restrict__ fc, float* __restrict__ f, int N0, int N, int o, int cubic) {
+    extern __shared__ float sm[];
+    float* line = sm;
+    float* tmp = sm + N;
+    const int yi = blockIdx.x + o, z = blockIdx.y;
+    const float* src = fc + ((size_t)z * N + yi) * N;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) line[x] = src[x];
+    __syncthreads();
+    if (cubic) iir_line_lds_T(line, tmp, N);
+    float* dst = f + ((size_t)z * N0 + (yi - o)) * N0;
+    for (int x = threadIdx.x; x < N0; x += blockDim.x) dst[x] = line[x + o];
+}
+
+// Transpose of k_sino_rows for the kept angle rows: s-direction prefilter
+// transpose, the FBP convolution (a symmetric Toeplitz matrix: its own
+// transpose), then the transpose of the edge replication (the columns left of
+// lb fold into column lb, those right of rb into rb) and of the placement.
+template <int EPT>
+__global__ void k_sino_rows_T(const float* __restrict__ Rc, float* __restrict__ R, SinoPrep sp, int cubic) {
+    extern __shared__ float2 smc[];
+    __shared__ float red[2][32];
+    const int N = sp.N;
+    const int a = blockIdx.x, z = blockIdx.y;
+    const int ai = a * sp.os;
+    const float* src = Rc + ((size_t)z * sp.P + ai) * N;
+    float* line = reinterpret_cast<float*>(smc + lds_slots(N));
+    float* tmp = line + N;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) line[x] = src[x];
+    __syncthreads();
+    if (cubic) iir_line_lds_T(line, tmp, N);
+    if (sp.H2) {
+        for (int e = threadIdx.x; e < N; e += blockDim.x)
+            smc[lidx(e)] = 2 * e < N ? make_float2(line[2 * e], line[2 * e + 1]) : make_float2(0.f, 0.f);
+        __syncthreads();
+        fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, -1);
+        r2c_post(smc, N, 1, N, sp.tw2N);
+        for (int k = threadIdx.x; k < N; k += blockDim.x) {
+            float2 v = smc[lidx(k)];
+            if (k == 0) smc[lidx(0)] = make_float2(v.x * sp.H2[0], v.y * sp.H2[N]);
+            else smc[lidx(k)] = make_float2(v.x * sp.H2[k], v.y * sp.H2[k]);
+        }
+        __syncthreads();
+        c2r_pre(smc, N, 1, N, sp.tw2N);
+        fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, +1);
+        for (int e = threadIdx.x; e < N / 2; e += blockDim.x) {
+            float2 v = smc[lidx(e)];
+            line[2 * e] = v.x;
+            line[2 * e + 1] = v.y;
+        }
+        __syncthreads();
+    }
+    // sums of the replicated columns [0, lb) and (rb, N)
+    const int lb = sp.lb, rb = sp.lb + sp.N0 - 1;
+    float sl = 0.f, sr = 0.f;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) {
+        if (x < lb) sl += line[x];
+        else if (x > rb) sr += line[x];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sl += __shfl_down(sl, off);
+        sr += __shfl_down(sr, off);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { red[0][wid] = sl; red[1][wid] = sr; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float tl = 0.f, tr = 0.f;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) { tl += red[0][w]; tr += red[1][w]; }
+        line[lb] += tl;
+        line[rb] += tr;
+    }
+    __syncthreads();
+    float* dst = R + ((size_t)z * sp.P0 + a) * sp.N0;
+    for (int x = threadIdx.x; x < sp.N0; x += blockDim.x) dst[x] = line[x + lb];
+}
+
+// Transpose of k_gather: per tile, the block clears an LDS copy of the source
+// box, adds every point's value times its tap weights into it (LDS atomics),
+// and adds the box into the destination grid (global atomics: neighbouring
+// boxes overlap).  Same tiles, logical block order and clamps as k_gather.
+template <int KIND>
+__global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTile* __restrict__ tiles) {
+    extern __shared__ float box[];
+    constexpr int PPT = kGatherPPT;
+    const int nt = blockDim.x;
+    const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
+    const int lbk = xcd_block(blockIdx.x, gridDim.x);
+    const int zg = lbk % nzg, tile = lbk / nzg;
+    const GTile* tb = tiles + (size_t)tile * g.group;
+    const GTile t0 = tb[0];
+    const int npts = t0.nr * t0.nc;
+    const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    for (int z = z0; z < z1; ++z) {
+        float val[PPT];
+        const float* V = g.val + (size_t)z * g.v_z + (size_t)t0.span * g.v_span;
+#pragma unroll
+        for (int e = 0; e < PPT; ++e) {
+            int p = threadIdx.x + e * nt;
+            val[e] = 0.f;
+            if (p < npts) {
+                int r = p / t0.nc, c = p - r * t0.nc;
+                float v = V[(size_t)(t0.r0 + r) * g.v_row + t0.c0 + c];
+                if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
+                val[e] = v;
+            }
+        }
+        for (int gi = 0; gi < g.group; ++gi) {
+            const GTile t = tb[gi];
+            if (t.bw <= 0) continue;   // uniform over the block
+            const int bs = t.bw | 1;
+            const int nbox = bs * t.bh;
+            for (int i = threadIdx.x; i < nbox; i += nt) box[i] = 0.f;
+            __syncthreads();
+            const float2* cbase = g.coord + (size_t)t.span * g.c_span;
+#pragma unroll 2
+            for (int e = 0; e < PPT; ++e) {
+                int p = threadIdx.x + e * nt;
+                if (p < npts && val[e] != 0.f) {
+                    int r = p / t.nc, cidx = p - r * t.nc;
+                    float2 c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
+                    if (c.x == c.x) {
+                        float fx = floorf(c.x), fy = floorf(c.y);
+                        float ax = c.x - fx, ay = c.y - fy;
+                        int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
+                        int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
+                        float* q = box + ly * bs + lx;
+                        const float v = val[e];
+                        if (g.cubic) {
+                            float wx[4], wy[4];
+                            bspline_w(ax, wx);
+                            bspline_w(ay, wy);
+#pragma unroll
+                            for (int b = 0; b < 4; ++b) {
+                                float vb = wy[b] * v;
+#pragma unroll
+                                for (int a2 = 0; a2 < 4; ++a2) atomicAdd(q + b * bs + a2, wx[a2] * vb);
+                            }
+                        } else {
+                            float* q1 = q + bs + 1;
+                            float* q2 = q1 + bs;
+                            atomicAdd(q1, (1.f - ay) * (1.f - ax) * v);
+                            atomicAdd(q1 + 1, (1.f - ay) * ax * v);
+                            atomicAdd(q2, ay * (1.f - ax) * v);
+                            atomicAdd(q2 + 1, ay * ax * v);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            float* D = g.dst + (size_t)z * g.dst_z + (size_t)t.span * g.dst_span;
+            const float inv = 1.0f / (float)t.bw;
+            const int nb = t.bw * t.bh;
+            for (int i = threadIdx.x; i < nb; i += nt) {
+                int yy = (int)(((float)i + 0.5f) * inv);
+                int xx = i - yy * t.bw;
+                float v = box[yy * bs + xx];
+                if (v != 0.f) unsafeAtomicAdd(D + (size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W), v);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Transpose of the extra samples added in the row pass: block (row, span, z)
+// scatters F[span][z][row][theta] of each of its extra entries into dst.
+__global__ void k_scatter_extra(const float* __restrict__ F, float* __restrict__ dst, int nz, int Nt, int Nr, int W,
+                                int H, const int* __restrict__ xrow, const float4* __restrict__ xent, int cubic) {
+    const int row = blockIdx.x, span = blockIdx.y, z = blockIdx.z;
+    const int lo = xrow[span * (Nr + 1) + row], hi = xrow[span * (Nr + 1) + row + 1];
+    const float* Fr = F + (((size_t)span * nz + z) * Nr + row) * (size_t)Nt;
+    float* D = dst + (size_t)z * W * H;
+    for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+        float4 q = xent[e];
+        float v = Fr[(int)q.x];
+        float x = q.y, y = q.z;
+        float fx = floorf(x), fy = floorf(y);
+        float ax = x - fx, ay = y - fy;
+        int ix = (int)fx, iy = (int)fy;
+        if (cubic) {
+            float wx[4], wy[4];
+            bspline_w(ax, wx);
+            bspline_w(ay, wy);
+            for (int b = 0; b < 4; ++b)
+                for (int a = 0; a < 4; ++a)
+                    unsafeAtomicAdd(D + (size_t)wrapi(iy - 1 + b, H) * W + wrapi(ix - 1 + a, W), wy[b] * wx[a] * v);
+        } else {
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a)
+                    unsafeAtomicAdd(D + (size_t)wrapi(iy + b, H) * W + wrapi(ix + a, W),
+                                    (b ? ay : 1.f - ay) * (a ? ax : 1.f - ax) * v);
+        }
+    }
+}
+
+void launch_zero(float* p, size_t n, hipStream_t s) {
+    size_t n4 = n / 4;
+    if (n4) {
+        size_t blocks = std::min<size_t>((n4 + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<float4*>(p), n4);
+    }
+    if (n % 4) hipLaunchKernelGGL(k_zero_tail, dim3(1), dim3(4), 0, s, p, n4 * 4, n);
+}
+
+void launch_scatter(const Scatter& g, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
+    if (ntiles <= 0) return;
+    dim3 grid((ntiles / g.group) * ((g.nz + g.zgroup - 1) / g.zgroup));
+    size_t lds = (size_t)g.box_floats * sizeof(float);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_scatter<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 1: hipLaunchKernelGGL(k_scatter<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 2: hipLaunchKernelGGL(k_scatter<2>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        default: hipLaunchKernelGGL(k_scatter<3>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+    }
+}
+
+void launch_scatter_extra(const float* F, float* dst, int nz, int Ntheta, int Nrho, int W, int H, const int* xrow,
+                          const float4* xent, bool cubic, hipStream_t s) {
+    if (!xrow) return;
+    hipLaunchKernelGGL(k_scatter_extra, dim3(Nrho, 3, nz), dim3(64), 0, s, F, dst, nz, Ntheta, Nrho, W, H, xrow, xent,
+                       (int)cubic);
+}
+
+void launch_col_iir_T(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s) {
+    dim3 block(256);
+    switch (chunk) {
+        case 32: hipLaunchKernelGGL(k_col_iir_T<32>, dim3((W + 255) / 256, (L + 31) / 32, nz), block, 0, s, in, out, L, W); break;
+        case 128: hipLaunchKernelGGL(k_col_iir_T<128>, dim3((W + 255) / 256, (L + 127) / 128, nz), block, 0, s, in, out, L, W); break;
+        default: hipLaunchKernelGGL(k_col_iir_T<64>, dim3((W + 255) / 256, (L + 63) / 64, nz), block, 0, s, in, out, L, W); break;
+    }
+}
+
+void launch_image_rows_T(const float* fc, float* f, int nz, int N0, int N, bool cubic, hipStream_t s) {
+    hipLaunchKernelGGL(k_image_rows_T, dim3(N0, nz), dim3(256), 2 * N * sizeof(float), s, fc, f, N0, N,
+                       N / 2 - N0 / 2, (int)cubic);
+}
+
+void launch_sino_rows_T(const float* Rc, float* R, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
+    int nt = threads_for(sp.N);
+    int ept = ept_for(sp.N, nt);
+    LP_DISPATCH_EPT(ept, k_sino_rows_T, dim3(sp.P0, nz), dim3(nt), sino_lds_bytes(sp.N), s, Rc, R, sp, (int)cubic);
 }
 
 }  // namespace lpk

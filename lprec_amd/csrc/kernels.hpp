@@ -44,10 +44,11 @@ struct RowPass {
     const float2* twh;        // W_h^j
     const float2* twN;        // W_Ntheta^k
     float2* S;                // [3][nz][h][Nrho]
+    int conj;                 // column pass multiplies by conj(fZ) (the transposed convolution)
 };
 // R2C along theta of the gathered log-polar grids F [3][nz][Nrho][Ntheta];
 // range [3][Nrho]: [lo, hi) theta bounds of the non-empty points of each row
-// (F is read only inside them, zero elsewhere); xrow/xent: extra samples that
+// (F is read only inside them, zero elsewhere; null = whole rows); xrow/xent: extra samples that
 // share a log-polar point with another one (added on top)
 void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const Src& src, const int* xrow,
                      const float4* xent, bool cubic, hipStream_t s);
@@ -87,13 +88,37 @@ constexpr int kGatherPPT = 8;   // points per thread: tiles hold <= 2048 points
 // kind: 0 image->LP, 1 sinogram->LP, 2 LP->polar, 3 LP->Cartesian (names the kernel instance)
 void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hipStream_t s);
 
-// ---- output gathers ---------------------------------------------------------
-// fwd: G -> user R [nz][P0][N0]; p2lp [P][N] (theta idx, rho idx), span [P]
-void launch_polar_out(const float* G, float* R, int nz, int Ntheta, int Nrho, int N, int N0, int P0, int os,
-                      int lb, const float2* p2lp, const unsigned char* span, bool cubic, hipStream_t s);
-// adj: sum over spans of G -> user f [nz][N0][N0]; cmapC [3][N][N]
-void launch_cart_out(const float* G, float* f, int nz, int Ntheta, int Nrho, int N, int N0,
-                     const float2* cmapC, bool cubic, hipStream_t s);
+// ---- transposes (exact adjoints of the stages above) -----------------------
+// Transpose of a Gather: the value at every output point of every tile (times
+// scale) is added into the taps of the gather's source grid, which is `dst`
+// here (accumulated with atomics: dst must be cleared first).
+struct Scatter {
+    float* dst;
+    long long dst_z, dst_span;
+    int W, H;
+    const float2* coord;
+    long long c_span;
+    int c_row, c_ro, c_co;
+    const float* scale;
+    int s_row;
+    const float* val;
+    long long v_z, v_span;
+    int v_row;
+    int nz, zgroup, group, cubic, box_floats;
+};
+void launch_scatter(const Scatter& g, const GTile* tiles, int ntiles, int kind, hipStream_t s);
+// transpose of the extra samples of launch_rows_fwd: F [3][nz][Nrho][Ntheta]
+// values scattered into dst [nz][H][W] at the extras' coordinates
+void launch_scatter_extra(const float* F, float* dst, int nz, int Ntheta, int Nrho, int W, int H,
+                          const int* xrow, const float4* xent, bool cubic, hipStream_t s);
+// transpose of launch_col_iir (same chunking)
+void launch_col_iir_T(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s);
+// transpose of launch_image_rows: fc [nz][N][N] -> f [nz][N0][N0]
+void launch_image_rows_T(const float* fc, float* f, int nz, int N0, int N, bool cubic, hipStream_t s);
+// transpose of launch_sino_rows: Rc [nz][P][N] -> R [nz][P0][N0]
+void launch_sino_rows_T(const float* Rc, float* R, int nz, const SinoPrep& sp, bool cubic, hipStream_t s);
+// p[0..n) = 0 (a kernel, so that it may be captured in a graph)
+void launch_zero(float* p, size_t n, hipStream_t s);
 
 size_t rows_lds_bytes(int ldm, int R);
 size_t cols_lds_bytes(int M);

@@ -9,9 +9,11 @@ case-insensitively (the reference's own tests pass 'None', which its
 initsadj.py:107 would reject), and failures raise instead of resetting the
 device.
 """
+import os
+
 import numpy as np
 
-from . import _lib
+from . import _lib, precache
 
 FILTERS = ("none", "ramp", "shepp-logan", "cosine", "cosine2", "hamming", "hann", "parzen")
 
@@ -85,9 +87,22 @@ class lpTransform:
         # initsgl.py:55 -- 'cubic' selects B-splines, anything else bilinear
         return self.interp_type == "cubic"
 
-    def precompute(self, flg):
-        """Build the operator tables on the host (lpTransform.py:19-31)."""
-        pre = _lib.precompute(self.N, self.Nproj, self.Nslices, self.cor, self.cubic, self.filter_type, bool(flg))
+    def precompute(self, flg, cache_dir=None):
+        """Build the operator tables on the host (lpTransform.py:19-31).
+
+        With ``cache_dir`` (or the environment variable LPREC_AMD_CACHE) the
+        tables are loaded from / stored to an on-disk cache keyed by the
+        geometry (lprec_amd.precache)."""
+        cache_dir = cache_dir or os.environ.get("LPREC_AMD_CACHE") or None
+        pre = None
+        if cache_dir:
+            pre = precache.load(cache_dir, self.N, self.Nproj, self.Nslices, self.cor, self.cubic,
+                                self.filter_type, bool(flg))
+        if pre is None:
+            pre = _lib.precompute(self.N, self.Nproj, self.Nslices, self.cor, self.cubic, self.filter_type,
+                                  bool(flg))
+            if cache_dir:
+                precache.store(cache_dir, self.N, self.Nproj, self.cor, self.cubic, self.filter_type, pre)
         self.glpars = pre["glpars"]
         if flg:
             self.fwdparsi = pre["fwdparsi"]
@@ -147,6 +162,24 @@ class lpTransform:
         Rp, stream = _device_array(R, (ns0, self.Nproj, self.N), "R", gpu)
         fp, _ = _device_array(f, (ns0, self.N, self.N), "f", gpu)
         h.adj(fp, Rp, ns0, stream)
+
+    def fwdp_transpose(self, f, R, gpu):
+        """f[Ns0, N, N] = F^T R[Ns0, Nproj, N]: the exact transpose of fwdp
+        (no reference counterpart; the gradient of fwdp)."""
+        h = self._handle(gpu)
+        ns0 = self._ns0(R, "R")
+        Rp, stream = _device_array(R, (ns0, self.Nproj, self.N), "R", gpu)
+        fp, _ = _device_array(f, (ns0, self.N, self.N), "f", gpu)
+        h.fwd_T(fp, Rp, ns0, stream)
+
+    def adjp_transpose(self, R, f, gpu):
+        """R[Ns0, Nproj, N] = A^T f[Ns0, N, N]: the exact transpose of adjp
+        (FBP filter included; no reference counterpart)."""
+        h = self._handle(gpu)
+        ns0 = self._ns0(f, "f")
+        fp, stream = _device_array(f, (ns0, self.N, self.N), "f", gpu)
+        Rp, _ = _device_array(R, (ns0, self.Nproj, self.N), "R", gpu)
+        h.adj_T(Rp, fp, ns0, stream)
 
     def close(self):
         for i, h in enumerate(self.clphandle):

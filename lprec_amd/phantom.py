@@ -44,3 +44,59 @@ def phantom_stack(nslices, N, n_ellipses, seed, inside=False):
     rng = np.random.default_rng(seed)
     return np.stack([rasterize(random_ellipses(n_ellipses, rng, inside), N)
                      for _ in range(nslices)]).astype(np.float32)
+
+
+def radon_scale(N, P0):
+    """Factor between lpTransform.fwdp output and line integrals in unit-disk
+    lengths: the fwd kernel constant sqrt(N os / P) pi / 4 / aR / sqrt(2)
+    (initsfwd.py:57-58) times aR, with P = os P0 (N = the padded grid)."""
+    return np.sqrt(N / P0) * np.pi / (4 * np.sqrt(2))
+
+
+def ellipse_sinogram(ell, N0, P0, cor=None):
+    """float64 [P0, N0] exact line integrals (unit-disk lengths) of the ellipse
+    set in lpTransform's geometry: angle a pi / P0, detector column x at
+    s = -1 + 2 (x + N/2 - int(cor)) / (N - 1) on the padded grid N, and
+    s = x cos(th) - y sin(th) (x along columns, y along rows)."""
+    import math
+    cor = N0 / 2 if cor is None else cor
+    N = int(math.ceil((N0 + abs(N0 / 2 - cor) * 2.0) / 16.0) * 16)
+    th = np.arange(P0) * np.pi / P0
+    s = -1 + 2 * (np.arange(N0) + N // 2 - int(cor)) / (N - 1)
+    w = np.stack([np.cos(th), -np.sin(th)], axis=1)          # [P0, 2]
+    out = np.zeros((P0, N0))
+    for A, a, b, x0, y0, psi in ell:
+        # in the ellipse's frame the line x.w = s meets the unit circle after
+        # scaling by (a, b): chord = 2 a b sqrt(h^2 - d^2) / h^2
+        wu = w[:, 0] * np.cos(psi) + w[:, 1] * np.sin(psi)
+        wv = -w[:, 0] * np.sin(psi) + w[:, 1] * np.cos(psi)
+        h2 = (a * wu) ** 2 + (b * wv) ** 2
+        d = s[None, :] - (x0 * w[:, 0] + y0 * w[:, 1])[:, None]
+        out += 2 * A * a * b * np.sqrt(np.clip(h2[:, None] - d * d, 0, None)) / h2[:, None]
+    return out
+
+
+def noisy_sinogram(p, photons, rng):
+    """Transmission noise: counts ~ Poisson(photons e^-p), clamped to >= 1,
+    returned as -log(counts / photons)."""
+    counts = np.maximum(rng.poisson(photons * np.exp(-np.asarray(p, np.float64))), 1)
+    return -np.log(counts / photons)
+
+
+def scan(nslices, N0, P0, n_ellipses, seed, photons=None, cor=None):
+    """-> (phantoms [ns, N0, N0] f32, sinograms [ns, P0, N0] f32 in fwdp units):
+    seeded ellipse phantoms inside the unit disk, their exact sinograms with
+    optional Poisson noise, scaled by radon_scale."""
+    import math
+    rng = np.random.default_rng(seed)
+    c = N0 / 2 if cor is None else cor
+    N = int(math.ceil((N0 + abs(N0 / 2 - c) * 2.0) / 16.0) * 16)
+    imgs, sinos = [], []
+    for _ in range(nslices):
+        ell = random_ellipses(n_ellipses, rng, inside=True)
+        imgs.append(rasterize(ell, N0))
+        p = ellipse_sinogram(ell, N0, P0, cor)
+        if photons:
+            p = noisy_sinogram(p, photons, rng)
+        sinos.append(p * radon_scale(N, P0))
+    return np.stack(imgs).astype(np.float32), np.stack(sinos).astype(np.float32)

@@ -12,7 +12,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 def declared():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(lp_[a-z_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(lp_[A-Za-z_]+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,160 @@
+"""Reconstruction methods on the GPU against the norms the reference's own
+tests record (run_test.py:22-48):
+
+* test_lpmethods.py:10-86 -- fixture ``f`` (N=512, 384 angles, 8 copies):
+  fbp 222.4238, grad 348.78424, cg 347.4832, em 328.2814, tv 324.60355;
+* test_foam.py:8-80 -- fixture ``Rfoam`` (N=2016, 299 angles, os=10):
+  fbp 602.86015, grad 1195.5931, cg 1139.5114, em 444.35743, tv 926.6204,
+  tve 1003.76623;
+* test_gpus_many.py:26-88 -- sin() sinograms (N=512, 768 angles, 32 slices),
+  tv, 100 iterations: 381.53;
+* test_gpus_many_map.py:35-100 -- the same data with cor = N/2 - 3 (padded
+  N = 528), grad with the Barzilai-Borwein step, 100 iterations: 678.1923.
+
+The reference asserts only ``norm - ref < dif`` (one-sided, dif 10 or 100);
+here the check is two-sided.  The reference's operators sample through
+texture hardware with 8-bit interpolation weights; this build uses exact
+weights, so the operators differ at the ~1e-3 level and iterates inherit
+that.  Tolerances: FBP, CG, EM and the TV family 1e-3 (CG on the noisy foam
+data 5e-3; observed 8e-5 .. 2.8e-3).  Gradient descent with the
+Barzilai-Borwein step is not monotone and amplifies operator differences:
+its tolerance is twice the spread the same run shows when the sinogram is
+perturbed by 1e-4 (relative), measured in the test itself, and at least 2e-3.
+"""
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-3
+TOL = {"cg": 5e-3}
+
+
+def _norm(x):
+    return float(np.linalg.norm(np.float64(x)))
+
+
+def _check(name, got, ref, tol=REL_TOL, failures=None):
+    print("%-28s norm %.5f  reference %.5f  rel %.2e" % (name, got, ref, abs(got - ref) / ref), flush=True)
+    ok = abs(got - ref) <= tol * ref
+    if failures is None:
+        assert ok, (name, got, ref)
+    elif not ok:
+        failures.append((name, got, ref))
+
+
+def _run(lp, method, init, tomo, it, reg):
+    from lprec_amd import lpmethods
+    return lpmethods.METHODS[method](lp, init, tomo, it, reg, 0)
+
+
+def _bb_tolerance(lp, init, tomo, it, reg, base):
+    """Relative norm spread of the BB gradient run under 1e-4 data perturbations."""
+    rng = np.random.default_rng(99)
+    spread = 0.0
+    for _ in range(2):
+        pert = tomo * (1 + 1e-4 * rng.choice([-1.0, 1.0], size=tomo.shape)).astype(np.float32)
+        spread = max(spread, abs(_norm(_run(lp, "grad", init, pert, it, reg)) - base) / base)
+    print("BB sensitivity: relative spread %.2e under 1e-4 data perturbation" % spread, flush=True)
+    return max(2e-3, 2 * spread)
+
+
+def _check_method(prefix, lp, m, init, tomo, it, reg, ref, bad):
+    got = _norm(_run(lp, m, init, tomo, it, reg))
+    tol = _bb_tolerance(lp, init, tomo, it, reg, got) if m == "grad" else TOL.get(m, REL_TOL)
+    _check(prefix + m, got, ref, tol=tol, failures=bad)
+
+
+def test_lpmethods_fixture_norms(torch_cuda):
+    import torch
+    from lprec_amd import lpTransform
+    N, P0, ns = 512, 384, 8
+    f = np.fromfile(os.path.join(GOLDEN, "f"), dtype=np.float32).reshape(1, N, N)
+    fa = np.repeat(f, ns, axis=0)
+    lp = lpTransform(N, P0, ns, "None", N / 2, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    Ra = torch.zeros((ns, P0, N), device="cuda")
+    lp.fwdp(Ra, torch.from_numpy(fa).cuda(), 0)
+    Ra = Ra.cpu().numpy()
+    ref = {"fbp": 222.4238, "grad": 348.78424, "cg": 347.4832, "em": 328.2814, "tv": 324.60355}
+    its = {"fbp": 0, "grad": 64, "cg": 16, "em": 64, "tv": 256}
+    reg = {"fbp": 0, "grad": -1, "cg": 0, "em": 0.01, "tv": 0.001}
+    bad = []
+    for m in ref:
+        init = np.zeros((ns, N, N), np.float32) + (1e-3 if m == "em" else 0)
+        _check_method("lpmethods/", lp, m, init, Ra, its[m], reg[m], ref[m], bad)
+    assert not bad, bad
+
+
+def test_foam_norms(torch_cuda):
+    from lprec_amd import lpTransform
+    N, P0 = 2016, 299
+    tomo = np.fromfile(os.path.join(GOLDEN, "Rfoam"), dtype=np.float32).reshape(1, N, P0).swapaxes(1, 2)
+    tomo = np.ascontiguousarray(tomo)
+    lp = lpTransform(N, P0, 1, "None", N / 2, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    ref = {"fbp": 602.86015, "grad": 1195.5931, "cg": 1139.5114, "em": 444.35743, "tv": 926.6204,
+           "tve": 1003.76623}
+    its = {"fbp": 0, "grad": 64, "cg": 16, "em": 64, "tv": 256, "tve": 256}
+    reg = {"fbp": 0, "grad": -1, "cg": 0, "em": 0.5, "tv": 0.001, "tve": 0.001}
+    bad = []
+    for m in ref:
+        init = np.zeros((1, N, N), np.float32) + (1e-3 if m == "em" else 0)
+        _check_method("foam/", lp, m, init, tomo, its[m], reg[m], ref[m], bad)
+    assert not bad, bad
+
+
+def _sin_stack(ns, P0, N):
+    R = np.float32(np.sin(np.arange(0, ns * P0 * N) / float(ns * P0)))
+    return R.reshape(ns, P0, N)
+
+
+def test_gpus_many_tv_norm(torch_cuda):
+    from lprec_amd import lpTransform
+    N, ns = 512, 32
+    P0 = int(3 * N / 2)
+    tomo = _sin_stack(ns, P0, N)
+    lp = lpTransform(N, P0, ns, "None", N / 2, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    out = _run(lp, "tv", np.zeros((ns, N, N), np.float32) + 1e-3, tomo, 100, 0.001)
+    _check("gpus_many/tv", _norm(out), 381.53)
+
+
+def test_gpus_many_map_grad_norm_with_centre_offset(torch_cuda):
+    from lprec_amd import lpTransform
+    N, ns = 512, 32
+    P0 = int(3 * N / 2)
+    tomo = _sin_stack(ns, P0, N)
+    lp = lpTransform(N, P0, ns, "None", int(N / 2) - 3, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    assert lp.clphandle[0].info()["N"] == 528
+    bad = []
+    _check_method("gpus_many_map/", lp, "grad", np.zeros((ns, N, N), np.float32) + 1e-3, tomo, 100, -1, 678.1923,
+                  bad)
+    assert not bad, bad
+
+
+def test_lprec_wrapper_matches_direct_fbp(torch_cuda):
+    import torch
+    from lprec_amd import lpTransform
+    from lprec_amd.recon import lprec
+    N, P0, ns = 256, 200, 5
+    rng = np.random.default_rng(21)
+    tomo = rng.standard_normal((ns, P0, N)).astype(np.float32)
+    out = lprec(tomo, N / 2, theta=np.arange(P0) * np.pi / P0, lpmethod="fbp", filter_name="shepp-logan",
+                chunk=2)
+    lp = lpTransform(N, P0, ns, "shepp-logan", N / 2, "cubic")
+    lp.precompute(0)
+    lp.initcmem(0, 0)
+    ref = torch.zeros((ns, N, N), device="cuda")
+    lp.adjp(ref, torch.from_numpy(tomo).cuda(), 0)
+    ref = ref.cpu().numpy()
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-6

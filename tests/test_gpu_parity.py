@@ -33,6 +33,7 @@ CASES = [
     (250, 300, 2, "ramp", 127.3, "cubic"),     # centre offset -> N = 256, os = 1
     (512, 384, 2, "hamming", 256, "cubic"),    # fixture geometry, os = 2
     (528, 401, 1, "parzen", 261, "cubic"),     # N = 528 = 2^4 3 11 (generic radix), odd P0
+    (224, 34, 2, "none", 112, "cubic"),        # os = 10 and 224 = 2^5 7: the foam fixture's regime
 ]
 
 

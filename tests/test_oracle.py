@@ -226,3 +226,15 @@ def test_fwd_matches_analytic_ellipse_radon():
     Ra = ellipse_radon(ell, N, P0) * lprec_radon_scale(N, P0)
     # pixelised phantom vs exact ellipses: discretisation error of the raster
     assert np.linalg.norm(R - Ra) / np.linalg.norm(Ra) < 0.03
+
+
+def test_product_sinogram_generator_matches_oracle_radon():
+    # lprec_amd.phantom.ellipse_sinogram (synthetic bench/test data) against
+    # the oracle's analytic Radon transform, centred and with a centre offset
+    from lprec_amd.phantom import ellipse_sinogram
+    rng = np.random.default_rng(12)
+    ell = random_ellipses(7, rng, inside=True)
+    assert np.allclose(ellipse_sinogram(ell, 128, 90), ellipse_radon(ell, 128, 90), atol=1e-12)
+    # cor = 61 -> padded N = 144, detector column x at s = -1 + 2 (x + 72 - 61) / 143
+    s = -1 + 2 * (np.arange(128) + 72 - 61) / 143
+    assert np.allclose(ellipse_sinogram(ell, 128, 90, cor=61), ellipse_radon(ell, 128, 90, s=s), atol=1e-12)
