@@ -171,6 +171,104 @@ def cpu_baseline(lp, f0, budget_s):
                       "(scipy.fft workers=%d), %.1f s, host CPU: %s" % (n, threads, dt, cpu)}
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def run_fbp(args):
+    """FBP (adjp with a filter) of 32 noisy slices at N0 = 2048, 1500 angles and
+    a centre offset drawn from U(-3, 3) px (padded grid N = 2064: FFT lengths
+    2^4 3 43 and 2^5 3 43), ramp and Parzen filters."""
+    import numpy as np
+    import torch
+    from lprec_amd import lpTransform
+    from lprec_amd.phantom import scan
+    N0, P0, ns = 2048, 1500, 32
+    cor = N0 / 2 + float(np.random.default_rng(2064).uniform(-3, 3))
+    f_true, R_host = scan(ns, N0, P0, 20, seed=2064, photons=1e4, cor=cor)
+    R = torch.from_numpy(R_host).cuda()
+    rec = torch.zeros((ns, N0, N0), device="cuda")
+    out = {}
+    for filt in ("ramp", "parzen"):
+        lp = lpTransform(N0, P0, ns, filt, cor, "cubic")
+        lp.precompute(0)
+        lp.initcmem(0, 0)
+        h = lp.clphandle[0]
+        g = h.info()
+        for _ in range(args.warmup):
+            lp.adjp(rec, R, 0)
+        torch.cuda.synchronize()
+        h.profile(True)
+        dt = _timed(lambda: lp.adjp(rec, R, 0), args.steps, 0)
+        stage_ms, ncalls = h.profile_read(1)
+        h.profile(False)
+        rate = ns * args.steps / dt
+        L = 4 * g["Nrho"] * g["Ntheta"]
+        b_alg = 4 * N0 * P0 + 16 * g["N"] * g["Nproj"] + 12 * L + 4 * N0 * N0
+        # reconstruction error against the phantom inside the disk (scale-free)
+        x = rec.cpu().numpy().astype(np.float64)
+        a = float(np.sum(x * f_true) / np.sum(x * x))
+        err = float(np.linalg.norm(a * x - f_true) / np.linalg.norm(f_true))
+        out[filt] = {"value": round(rate, 2), "ms_per_call": round(dt / args.steps * 1e3, 3),
+                     "B_alg_bytes_per_slice": int(b_alg), "roofline_frac": round(b_alg * rate / HBM_PEAK, 4),
+                     "rel_error_vs_phantom": round(err, 4),
+                     "stage_ms_per_call": {k: round(v / max(ncalls, 1), 3) for k, v in stage_ms.items()}}
+        lp.close()
+    line = {"metric": "FBP slices/s at N0=2048, 1500 proj, centre offset (N=2064), 32 slices",
+            "value": out["ramp"]["value"], "unit": "slices/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic: 20-ellipse phantoms, analytic sinograms, Poisson 1e4 photons",
+            "config": {"workload": "FBP ramp/parzen, N0=2048, Nproj=1500, cor=%.3f (N=%d), 32 slices"
+                                   % (cor, g["N"]), "N0": N0, "Nproj": P0, "cor": cor},
+            "filters": out}
+    print(json.dumps(line), flush=True)
+
+
+def run_multiscan(args):
+    """96 scans, N0 from {512, 1024, 2048}, Nproj from {360, 720, 1500}, 1-16
+    slices each: fwd+adj of every scan per step, one graph launch per step."""
+    import numpy as np
+    import torch
+    from lprec_amd.batch import MultiScan, bytes_per_slice, random_scans
+    from lprec_amd.phantom import phantom_stack
+    scans = random_scans(96, seed=96)
+    t0 = time.perf_counter()
+    ms = MultiScan(scans, cache_dir=os.environ.get("LPREC_AMD_CACHE"))
+    t_init = time.perf_counter() - t0
+    for i, s in enumerate(scans):
+        ms.scan_views(i)[0].copy_(torch.from_numpy(phantom_stack(s.nslices, s.N0, 12, seed=i)))
+    tuned = None
+    if args.autotune:
+        tuned = {k: v[0] for k, v in ms.autotune(reps=2, sweeps=1).items()}
+    ms.capture()
+    nsl = ms.nslices
+    b_alg = sum(bytes_per_slice(s.N0, s.Nproj) * s.nslices for s in scans)
+    t_graph = _timed(ms.replay, args.steps, args.warmup)
+    t_conc = _timed(lambda: ms.step_eager(True), args.steps, args.warmup)
+    t_seq = _timed(lambda: ms.step_eager(False), args.steps, args.warmup)
+    rate = nsl * args.steps / t_graph
+    line = {"metric": "uneven multi-scan fwd+adj slices/s (96 scans, one graph launch per step)",
+            "value": round(rate, 2), "unit": "slices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(t_graph / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic: seeded random-ellipse phantoms",
+            "config": {"workload": "96 scans, N0 in {512,1024,2048}, Nproj in {360,720,1500}, 1-16 slices",
+                       "scans": len(scans), "slices": nsl, "geometries": len(ms.groups)},
+            "scans_per_s": round(len(scans) * args.steps / t_graph, 2),
+            "pipeline_roofline_frac": round(b_alg * args.steps / t_graph / HBM_PEAK, 4),
+            "eager_concurrent_slices_per_s": round(nsl * args.steps / t_conc, 2),
+            "eager_sequential_slices_per_s": round(nsl * args.steps / t_seq, 2),
+            "init_s": round(t_init, 1), "schedules": tuned}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,12 +280,16 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup)")
+    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--pmc-summary", default="", metavar="DIR",
                     help="print calibrated per-stage traffic from rocprofv3 --pmc runs under DIR and exit")
     ap.add_argument("--pmc-copy-mib", type=int, default=4096)
+    ap.add_argument("--workload", default="fwdadj", choices=("fwdadj", "fbp", "multiscan"),
+                    help="fwdadj: the headline line; fbp: FBP at the centre-offset geometry (ramp and "
+                         "parzen); multiscan: 96 uneven scans in one graph launch")
+    ap.add_argument("--autotune", action="store_true", help="multiscan: autotune every geometry first")
     args = ap.parse_args()
     if args.pmc_summary:
         print(json.dumps(pmc_summary(args.pmc_summary, args.pmc_copy_mib), indent=1))
@@ -202,6 +304,10 @@ def main():
     from lprec_amd.phantom import phantom_stack
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "fbp":
+        return run_fbp(args)
+    if args.workload == "multiscan":
+        return run_multiscan(args)
     if args.copy_ref:
         # known-size copy: reads and writes copy_ref MiB per launch
         n = args.copy_ref * (1 << 20) // 4
@@ -218,7 +324,7 @@ def main():
     lp.precompute(1)
     lp.initcmem(1, 0)
     h = lp.clphandle[0]
-    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3}
+    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4}
     for kv in filter(None, args.tune.split(",")):
         k, v = kv.split("=")
         h.set_tuning(knobs[k], int(v))

@@ -57,7 +57,7 @@ struct lp_handle {
     int N = 0, N0 = 0, Nt = 0, Nr = 0, P = 0, P0 = 0, Ns = 0, cor = 0, os = 1, h = 0;
     bool cubic = true;
     bool has_fwd = false, has_adj = false, has_filter = false;
-    int R = 8, zgroup = 2, colchunk = 64, gzgroup = 2;
+    int R = 8, zgroup = 2, colchunk = 64, gzgroup = 2, fft16 = 1;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -591,6 +591,7 @@ static lpk::RowPass row_pass(lp_handle* h, int nz) {
     rp.twN = h->tw[h->N];
     rp.S = h->d_S;
     rp.conj = 0;
+    rp.fft16 = h->fft16;
     return rp;
 }
 
@@ -860,6 +861,10 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             if (value < 1) return h->fail(LP_ERR_ARG, "slices per gather block must be >= 1");
             h->gzgroup = value;
             return LP_OK;
+        case 4:
+            if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "FFT kernel family must be 0 or 1");
+            h->fft16 = value;
+            return LP_OK;
     }
     return h->fail(LP_ERR_ARG, "unknown tuning key");
 }
@@ -926,6 +931,7 @@ int lp_get_tuning(const lp_handle* h, int key, int* value) {
         case 1: *value = h->zgroup; return LP_OK;
         case 2: *value = h->colchunk; return LP_OK;
         case 3: *value = h->gzgroup; return LP_OK;
+        case 4: *value = h->fft16; return LP_OK;
     }
     return LP_ERR_ARG;
 }

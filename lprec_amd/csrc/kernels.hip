@@ -872,6 +872,7 @@ void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hi
 // middle radix of h = 16 * MID * 16, or 0 when the register kernels do not apply
 static int rows16_mid(const RowPass& rp) {
     int h = rp.h;
+    if (!rp.fft16) return 0;
     if (h < 256 || h > 4096 || (h & (h - 1))) return 0;
     if ((long long)rp.R * (h / 16) > 1024) return 0;
     return h / 256;
@@ -938,7 +939,7 @@ static int cols16_mid(int M) {
 }
 
 void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s) {
-    if (int mid = cols16_mid(rp.Nrho)) {
+    if (int mid = rp.fft16 ? cols16_mid(rp.Nrho) : 0) {
         const int T16 = rp.Nrho / 16;
         // column 0 (packed theta-DC / Nyquist pair) on the generic kernel
         {

@@ -45,6 +45,7 @@ struct RowPass {
     const float2* twN;        // W_Ntheta^k
     float2* S;                // [3][nz][h][Nrho]
     int conj;                 // column pass multiplies by conj(fZ) (the transposed convolution)
+    int fft16;                // use the radix-16 register kernels where the length allows
 };
 // R2C along theta of the gathered log-polar grids F [3][nz][Nrho][Ntheta];
 // range [3][Nrho]: [lo, hi) theta bounds of the non-empty points of each row

@@ -25,9 +25,10 @@ def random_ellipses(n, rng, inside=False):
     return np.array(out)
 
 
-def rasterize(ell, N):
-    """float64 [N, N] image of the ellipse set, masked to the unit disk."""
-    lin = np.linspace(-1, 1, N)
+def rasterize(ell, N, lin=None):
+    """float64 [N, N] image of the ellipse set, masked to the unit disk, at
+    pixel coordinates ``lin`` (default linspace(-1, 1, N))."""
+    lin = np.linspace(-1, 1, N) if lin is None else lin
     X, Y = np.meshgrid(lin, lin)
     img = np.zeros((N, N))
     for A, a, b, x0, y0, psi in ell:
@@ -92,9 +93,12 @@ def scan(nslices, N0, P0, n_ellipses, seed, photons=None, cor=None):
     c = N0 / 2 if cor is None else cor
     N = int(math.ceil((N0 + abs(N0 / 2 - c) * 2.0) / 16.0) * 16)
     imgs, sinos = [], []
+    # the user's image is the centre N0 x N0 block of the padded grid, whose
+    # pixels sit at linspace(-1, 1, N)
+    lin = -1 + 2 * (np.arange(N0) + N // 2 - N0 // 2) / (N - 1)
     for _ in range(nslices):
         ell = random_ellipses(n_ellipses, rng, inside=True)
-        imgs.append(rasterize(ell, N0))
+        imgs.append(rasterize(ell, N0, lin))
         p = ellipse_sinogram(ell, N0, P0, cor)
         if photons:
             p = noisy_sinogram(p, photons, rng)
