@@ -280,7 +280,7 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16)")
+    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16, boxcap)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--pmc-summary", default="", metavar="DIR",
@@ -320,14 +320,15 @@ def main():
         return
     N0, P0, ns = args.N, args.nproj, args.slices
     cor = N0 / 2
-    lp = lpTransform(N0, P0, ns, "none", cor, "cubic")
-    lp.precompute(1)
-    lp.initcmem(1, 0)
-    h = lp.clphandle[0]
-    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4}
+    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5}
+    sched = {}
     for kv in filter(None, args.tune.split(",")):
         k, v = kv.split("=")
-        h.set_tuning(knobs[k], int(v))
+        sched[k] = int(v)
+    lp = lpTransform(N0, P0, ns, "none", cor, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0, schedule=sched)
+    h = lp.clphandle[0]
     g = h.info()
     f_host = phantom_stack(ns, N0, 12, seed=2048)
     f = torch.from_numpy(f_host).cuda()

@@ -94,7 +94,9 @@ int lp_info(const lp_handle* h, int64_t* info);
  * prefilter column chunk (32, 64 or 128), key 3 = slices per block in the
  * tiled gathers (>=1), key 4 = FFT kernel family (1 = radix-16 register
  * kernels where the length is 16*k*16 with k in {1,2,4,8,16}, 0 = the
- * mixed-radix LDS kernels everywhere).  Returns LP_ERR_ARG for bad values. */
+ * mixed-radix LDS kernels everywhere), key 5 = LDS budget of a gather block
+ * in floats (1024..32768; tiles are split until their source box fits; only
+ * before lp_init_fwd / lp_init_adj).  Returns LP_ERR_ARG for bad values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 

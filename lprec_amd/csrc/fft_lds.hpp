@@ -185,12 +185,17 @@ __device__ void fft_stage_generic(float2* buf, int ldm, int nlines, int M, int N
             int e = tstep * (k + p * Ns);  // exponent of W_M per input index q
             const int src = b * ldm + i;
             float2 acc = make_float2(0.f, 0.f);
+            // W^(e q) by recurrence, re-read from the table every 8 terms
+            // (at most 7 chained products: ~4e-7 relative)
+            const float2 w1 = twiddle(tw, e, dir);
+            float2 w = make_float2(1.f, 0.f);
             int ex = 0;
             for (int q = 0; q < R; ++q) {
-                float2 w = twiddle(tw, ex, dir);
+                if ((q & 7) == 0) w = twiddle(tw, ex, dir);
                 float2 x = buf[lidx(src + q * nbf)];
                 acc.x += x.x * w.x - x.y * w.y;
                 acc.y += x.x * w.y + x.y * w.x;
+                w = cmul(w, w1);
                 ex += e;
                 if (ex >= M) ex -= M;
             }
@@ -234,25 +239,26 @@ __device__ void fft_lds(float2* buf, int ldm, int nlines, const FftPlan& plan, c
 // Real-to-complex post-processing.  Line b holds Z = FFT_h(z) with
 // z[n] = x[2n] + i x[2n+1] (N = 2h real samples).  Overwrites it with
 // X[0..h-1] of the length-N real DFT, X[0] (real) and X[h] (real) packed as
-// X[0] + i X[h] in slot 0.  twN[k] = exp(-2 pi i k / N).
-__device__ inline void r2c_post(float2* buf, int ldm, int nlines, int h, const float2* __restrict__ twN) {
+// X[0] + i X[h] in slot 0.  twN[k] = exp(-2 pi i k / N).  idx(b, k) is the
+// LDS slot of element k of line b.
+template <class Idx>
+__device__ inline void r2c_post_at(float2* buf, int nlines, int h, const float2* __restrict__ twN, Idx idx) {
     const int npairs = h / 2 + 1;
     for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
         int b = t / npairs, k = t - b * npairs;
-        const int L = b * ldm;
         if (k == 0) {
-            float2 z = buf[lidx(L)];
-            buf[lidx(L)] = make_float2(z.x + z.y, z.x - z.y);
+            float2 z = buf[idx(b, 0)];
+            buf[idx(b, 0)] = make_float2(z.x + z.y, z.x - z.y);
         } else {
             int kk = h - k;
-            float2 A = buf[lidx(L + k)], B = buf[lidx(L + kk)];
+            float2 A = buf[idx(b, k)], B = buf[idx(b, kk)];
             float2 Bc = cconj(B);
             float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
             float2 D = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
             float2 O = make_float2(D.y, -D.x);  // D / i
             float2 WO = cmul(twN[k], O);
-            buf[lidx(L + k)] = cadd(E, WO);
-            if (kk != k) buf[lidx(L + kk)] = cconj(csub(E, WO));
+            buf[idx(b, k)] = cadd(E, WO);
+            if (kk != k) buf[idx(b, kk)] = cconj(csub(E, WO));
         }
     }
     __syncthreads();
@@ -260,32 +266,40 @@ __device__ inline void r2c_post(float2* buf, int ldm, int nlines, int h, const f
 
 // Inverse of r2c_post: turns the packed half spectrum X into Z' such that the
 // unnormalised inverse FFT_h of Z' gives N * x (even samples in .x, odd in .y).
-__device__ inline void c2r_pre(float2* buf, int ldm, int nlines, int h, const float2* __restrict__ twN) {
+template <class Idx>
+__device__ inline void c2r_pre_at(float2* buf, int nlines, int h, const float2* __restrict__ twN, Idx idx) {
     const int npairs = h / 2 + 1;
     for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
         int b = t / npairs, k = t - b * npairs;
-        const int L = b * ldm;
         if (k == 0) {
-            float2 p = buf[lidx(L)];  // X0 + i Xh
-            buf[lidx(L)] = make_float2(p.x + p.y, p.x - p.y);
+            float2 p = buf[idx(b, 0)];  // X0 + i Xh
+            buf[idx(b, 0)] = make_float2(p.x + p.y, p.x - p.y);
         } else {
             int kk = h - k;
-            float2 P = buf[lidx(L + k)], Q = buf[lidx(L + kk)];
+            float2 P = buf[idx(b, k)], Q = buf[idx(b, kk)];
             float2 S = cadd(P, cconj(Q));
             float2 D = csub(P, cconj(Q));
             float2 w = twN[k];
             float2 iwc = make_float2(w.y, w.x);  // i * conj(w)
-            buf[lidx(L + k)] = cadd(S, cmul(iwc, D));
+            buf[idx(b, k)] = cadd(S, cmul(iwc, D));
             if (kk != k) {
                 // Z'[h-k] = conj(S) + i conj(W^{h-k}) (Q - conj P), conj(W^{h-k}) = -W^k
                 float2 S2 = cconj(S);
                 float2 D2 = cconj(make_float2(-D.x, -D.y));  // Q - conj(P) = -conj(D)
                 float2 imw = make_float2(w.y, -w.x);          // -i * w
-                buf[lidx(L + kk)] = cadd(S2, cmul(imw, D2));
+                buf[idx(b, kk)] = cadd(S2, cmul(imw, D2));
             }
         }
     }
     __syncthreads();
+}
+
+// lines at lidx(b * ldm + k)
+__device__ inline void r2c_post(float2* buf, int ldm, int nlines, int h, const float2* __restrict__ twN) {
+    r2c_post_at(buf, nlines, h, twN, [=](int b, int k) { return lidx(b * ldm + k); });
+}
+__device__ inline void c2r_pre(float2* buf, int ldm, int nlines, int h, const float2* __restrict__ twN) {
+    c2r_pre_at(buf, nlines, h, twN, [=](int b, int k) { return lidx(b * ldm + k); });
 }
 
 // ---------------------------------------------------------------------------

@@ -57,7 +57,7 @@ struct lp_handle {
     int N = 0, N0 = 0, Nt = 0, Nr = 0, P = 0, P0 = 0, Ns = 0, cor = 0, os = 1, h = 0;
     bool cubic = true;
     bool has_fwd = false, has_adj = false, has_filter = false;
-    int R = 8, zgroup = 2, colchunk = 64, gzgroup = 2, fft16 = 1;
+    int R = 8, zgroup = 2, colchunk = 64, gzgroup = 4, fft16 = 1;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -84,7 +84,7 @@ struct lp_handle {
     lpk::GTile* d_tilesC = nullptr;   // LP -> Cartesian (groups of 3 spans)
     int ntilesF = 0, ntilesP = 0, ntilesA = 0, ntilesC = 0;
     int boxF = 0, boxP = 0, boxA = 0, boxC = 0;
-    int box_cap = 12288;  // floats of LDS per gather block
+    int box_cap = 6144;   // floats of LDS per gather block (24 KiB: 6 blocks per CU)
     int stop_after = 0;   // stage-wise testing: return after this stage (0 = never)
 
     float* d_Rc = nullptr;
@@ -865,6 +865,12 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "FFT kernel family must be 0 or 1");
             h->fft16 = value;
             return LP_OK;
+        case 5:
+            if (h->has_fwd || h->has_adj)
+                return h->fail(LP_ERR_ARG, "the gather box budget is fixed once lp_init_fwd / lp_init_adj ran");
+            if (value < 1024 || value > 32768) return h->fail(LP_ERR_ARG, "gather box budget must be in [1024, 32768] floats");
+            h->box_cap = value;
+            return LP_OK;
     }
     return h->fail(LP_ERR_ARG, "unknown tuning key");
 }
@@ -932,6 +938,7 @@ int lp_get_tuning(const lp_handle* h, int key, int* value) {
         case 2: *value = h->colchunk; return LP_OK;
         case 3: *value = h->gzgroup; return LP_OK;
         case 4: *value = h->fft16; return LP_OK;
+        case 5: *value = h->box_cap; return LP_OK;
     }
     return LP_ERR_ARG;
 }

@@ -332,11 +332,72 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // LDS-tiled gather (the interp kernel of main_kernels.cuh:64-84 for all four
 // grids): the block stages the source box of its output tile with coalesced
 // row loads, then evaluates every sample of the tile from LDS.  A block owns
-// `group` tiles that share one output tile (the three spans of the
-// LP -> Cartesian gather, lpRgpu.cu:328, summed in span order).
+// GROUP tiles that share one output tile (the three spans of the
+// LP -> Cartesian gather, lpRgpu.cu:328, summed in span order); with
+// GROUP = 1 each sample is stored as soon as it is computed.
 // KIND only names the instance (0 image->LP, 1 sinogram->LP, 2 LP->polar,
 // 3 LP->Cartesian) so that profiles separate the four gathers.
-template <int KIND>
+
+// Stage the box [x0, x0+bw) x [y0, y0+bh) of the periodic grid S [H][W] into
+// LDS rows of stride bw|1, 8 loads in flight per lane, 32-bit offsets from
+// the block-uniform base.
+__device__ __forceinline__ void load_box(float* box, const float* __restrict__ S, const GTile& t, int W, int H) {
+    constexpr int UL = 8;
+    const int nt = blockDim.x;
+    const int nbox = t.bw * t.bh;
+    const int bs = t.bw | 1;
+    // (i + 0.5) / bw in float is exact enough for i, bw < 2^20
+    const float inv = 1.0f / (float)t.bw;
+    for (int i0 = threadIdx.x; i0 < nbox; i0 += UL * nt) {
+        float v[UL];
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            int i = i0 + u * nt;
+            v[u] = 0.f;
+            if (i < nbox) {
+                int yy = (int)(((float)i + 0.5f) * inv);
+                int xx = i - yy * t.bw;
+                unsigned off = (unsigned)wrapi(t.y0 + yy, H) * (unsigned)W + (unsigned)wrapi(t.x0 + xx, W);
+                v[u] = S[off];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            int i = i0 + u * nt;
+            if (i < nbox) {
+                int yy = (int)(((float)i + 0.5f) * inv);
+                box[yy * bs + i - yy * t.bw] = v[u];
+            }
+        }
+    }
+}
+
+// One sample of the staged box at source coordinate c (valid).
+__device__ __forceinline__ float box_sample(const float* box, const GTile& t, float2 c, int cubic) {
+    float fx = floorf(c.x), fy = floorf(c.y);
+    float ax = c.x - fx, ay = c.y - fy;
+    const int bs = t.bw | 1;
+    int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
+    int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
+    const float* q = box + ly * bs + lx;
+    if (cubic) {
+        float wx[4], wy[4];
+        bspline_w(ax, wx);
+        bspline_w(ay, wy);
+        float v = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float* qr = q + b * bs;
+            v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
+        }
+        return v;
+    }
+    const float* q1 = q + bs + 1;
+    const float* q2 = q1 + bs;
+    return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
+}
+
+template <int KIND, int GROUP>
 __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile* __restrict__ tiles) {
     extern __shared__ float box[];
     constexpr int PPT = kGatherPPT;
@@ -346,46 +407,19 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
     const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int zg = lb % nzg, tile = lb / nzg;
-    const GTile* tb = tiles + (size_t)tile * g.group;
+    const GTile* tb = tiles + (size_t)tile * GROUP;
     const GTile t0 = tb[0];
     const int npts = t0.nr * t0.nc;
     const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
     for (int z = z0; z < z1; ++z) {
-        float acc[PPT];
+        float* O = g.out + (size_t)z * g.o_z + (size_t)t0.span * g.o_span;
+        float acc[GROUP > 1 ? PPT : 1];
 #pragma unroll
-        for (int e = 0; e < PPT; ++e) acc[e] = 0.f;
-        for (int gi = 0; gi < g.group; ++gi) {
+        for (int e = 0; e < (GROUP > 1 ? PPT : 1); ++e) acc[e] = 0.f;
+#pragma unroll 1
+        for (int gi = 0; gi < GROUP; ++gi) {
             const GTile t = tb[gi];
-            if (t.bw > 0) {
-                // flattened box, 8 independent loads per thread in flight;
-                // (i + 0.5) / bw in float is exact enough for i, bw < 2^20
-                const float* S = g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span;
-                const int nbox = t.bw * t.bh;
-                const int bs = t.bw | 1;  // odd LDS row stride
-                const float inv = 1.0f / (float)t.bw;
-                constexpr int UL = 8;
-                for (int i0 = threadIdx.x; i0 < nbox; i0 += UL * nt) {
-                    float v[UL];
-#pragma unroll
-                    for (int u = 0; u < UL; ++u) {
-                        int i = i0 + u * nt;
-                        v[u] = 0.f;
-                        if (i < nbox) {
-                            int yy = (int)(((float)i + 0.5f) * inv);
-                            int xx = i - yy * t.bw;
-                            v[u] = S[(size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W)];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < UL; ++u) {
-                        int i = i0 + u * nt;
-                        if (i < nbox) {
-                            int yy = (int)(((float)i + 0.5f) * inv);
-                            box[yy * bs + i - yy * t.bw] = v[u];
-                        }
-                    }
-                }
-            }
+            if (t.bw > 0) load_box(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
             __syncthreads();
             const float2* cbase = g.coord + (size_t)t.span * g.c_span;
 #pragma unroll 2
@@ -394,44 +428,27 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
                 if (p < npts) {
                     int r = p / t.nc, cidx = p - r * t.nc;
                     float2 c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
-                    if (c.x == c.x) {
-                        float fx = floorf(c.x), fy = floorf(c.y);
-                        float ax = c.x - fx, ay = c.y - fy;
-                        const int bs = t.bw | 1;
-                        int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
-                        int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
-                        const float* q = box + ly * bs + lx;
-                        float v;
-                        if (g.cubic) {
-                            float wx[4], wy[4];
-                            bspline_w(ax, wx);
-                            bspline_w(ay, wy);
-                            v = 0.f;
-#pragma unroll
-                            for (int b = 0; b < 4; ++b) {
-                                const float* qr = q + b * bs;
-                                v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
-                            }
-                        } else {
-                            const float* q1 = q + bs + 1;
-                            const float* q2 = q1 + bs;
-                            v = (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
-                        }
+                    float v = (c.x == c.x) ? box_sample(box, t, c, g.cubic) : 0.f;
+                    if (GROUP == 1) {
+                        if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
+                        O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + cidx] = v;
+                    } else {
                         acc[e] += v;
                     }
                 }
             }
             __syncthreads();
         }
-        float* O = g.out + (size_t)z * g.o_z + (size_t)t0.span * g.o_span;
+        if (GROUP > 1) {
 #pragma unroll
-        for (int e = 0; e < PPT; ++e) {
-            int p = threadIdx.x + e * nt;
-            if (p < npts) {
-                int r = p / t0.nc, c = p - r * t0.nc;
-                float v = acc[e];
-                if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
-                O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + c] = v;
+            for (int e = 0; e < PPT; ++e) {
+                int p = threadIdx.x + e * nt;
+                if (p < npts) {
+                    int r = p / t0.nc, c = p - r * t0.nc;
+                    float v = acc[e];
+                    if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
+                    O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + c] = v;
+                }
             }
         }
     }
@@ -684,8 +701,12 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
                                                      Src src, const int* __restrict__ xrow,
                                                      const float4* __restrict__ xent, int cubic) {
     extern __shared__ float2 smc[];
-    constexpr int H = 256 * MID, T16 = 16 * MID, RS = H + H / 16;   // row stride in LDS slots
+    // row stride in LDS slots: the padded row plus 32/R slots.  The transposed
+    // accesses rr * RS + lidx(f) of a half-wave cover R rows x 32/R columns,
+    // which then land on 32 distinct 8-byte bank pairs
+    constexpr int H = 256 * MID, T16 = 16 * MID;
     const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
+    const int RS = H + H / 16 + 32 / R;
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
     // logical order: neighbouring row blocks (which share the 128-byte lines
     // of the transposed spectra) run behind the same L2
@@ -744,8 +765,7 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
         __syncthreads();
-        // lidx(r * H + k) == r * RS + lidx(k): the shared helpers see the same layout
-        r2c_post(smc, H, R, H, rp.twN);
+        r2c_post_at(smc, R, H, rp.twN, [=](int b, int k) { return b * RS + lidx(k); });
         float2* out = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
         for (int qq = threadIdx.x; qq < R * H; qq += blockDim.x) {
             int f = qq / R, rr = qq - f * R;
@@ -758,8 +778,9 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
 template <int MID>
 __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restrict__ G) {
     extern __shared__ float2 smc[];
-    constexpr int H = 256 * MID, T16 = 16 * MID, RS = H + H / 16;
+    constexpr int H = 256 * MID, T16 = 16 * MID;
     const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
+    const int RS = H + H / 16 + 32 / R;
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
     // logical order: neighbouring row blocks (which share the 128-byte lines
     // of the transposed spectra) run behind the same L2
@@ -775,7 +796,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
             R * H, [&](int qq) { int f = qq / R, rr = qq - f * R; return in[(size_t)f * Nr + rr]; },
             [&](int qq, float2 v) { int f = qq / R, rr = qq - f * R; smc[rr * RS + lidx(f)] = v; });
         __syncthreads();
-        c2r_pre(smc, H, R, H, rp.twN);
+        c2r_pre_at(smc, R, H, rp.twN, [=](int b, int k) { return b * RS + lidx(k); });
         float2 v[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
@@ -861,11 +882,13 @@ void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hi
     if (ntiles <= 0) return;
     dim3 grid((ntiles / g.group) * ((g.nz + g.zgroup - 1) / g.zgroup));
     size_t lds = (size_t)g.box_floats * sizeof(float);
+    if (kind == 3 && g.group != 3) return;   // the Cartesian gather always sums three spans
+    if (kind != 3 && g.group != 1) return;
     switch (kind) {
-        case 0: hipLaunchKernelGGL(k_gather<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-        case 1: hipLaunchKernelGGL(k_gather<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-        case 2: hipLaunchKernelGGL(k_gather<2>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-        default: hipLaunchKernelGGL(k_gather<3>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 0: hipLaunchKernelGGL((k_gather<0, 1>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 1: hipLaunchKernelGGL((k_gather<1, 1>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 2: hipLaunchKernelGGL((k_gather<2, 1>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        default: hipLaunchKernelGGL((k_gather<3, 3>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
     }
 }
 
@@ -877,7 +900,9 @@ static int rows16_mid(const RowPass& rp) {
     if ((long long)rp.R * (h / 16) > 1024) return 0;
     return h / 256;
 }
-static size_t rows16_lds(const RowPass& rp) { return (size_t)rp.R * (rp.h + rp.h / 16) * sizeof(float2) + 64; }
+static size_t rows16_lds(const RowPass& rp) {
+    return (size_t)rp.R * (rp.h + rp.h / 16 + 32 / rp.R) * sizeof(float2) + 64;
+}
 
 #define LP_DISPATCH_MID(mid, KER, grid, block, lds, stream, ...)                                                  \
     do {                                                                                                         \

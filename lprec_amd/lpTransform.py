@@ -111,8 +111,11 @@ class lpTransform:
         self.adjparsi = pre["adjparsi"]
         self.adjparamsf = pre["adjparamsf"]
 
-    def initcmem(self, flg, gpu):
-        """Create the per-GPU handle and upload the tables (lpTransform.py:34-40)."""
+    def initcmem(self, flg, gpu, schedule=None):
+        """Create the per-GPU handle and upload the tables (lpTransform.py:34-40).
+
+        ``schedule``: optional {knob: value} applied before the tables are
+        uploaded (see lprec_amd.tune.KNOBS; "boxcap" only takes effect here)."""
         if not 0 <= int(gpu) < len(self.clphandle):
             raise ValueError("gpu index must be in [0, 16)")
         if not hasattr(self, "glpars"):
@@ -120,6 +123,10 @@ class lpTransform:
         if flg and not self._fwd_ready:
             raise ValueError("precompute(1) is required before initcmem(1, gpu)")
         h = _lib.Handle(self.glpars, gpu)
+        if schedule:
+            from .tune import KNOBS
+            for k, v in schedule.items():
+                h.set_tuning(KNOBS[k], int(v))
         if flg:
             h.init_fwd(self.fwdparsi, self.fwdparamsf)
         h.init_adj(self.adjparsi, self.adjparamsf)

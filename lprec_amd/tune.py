@@ -16,7 +16,9 @@ import torch
 
 from ._lib import LprecError
 
-KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4}
+KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5}
+# knobs that can change after the tables are built (boxcap is fixed at init)
+RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16")
 SPACE = {
     "rows": (4, 8, 16),
     "zgroup": (1, 2, 4),
@@ -27,7 +29,7 @@ SPACE = {
 
 
 def get_schedule(h):
-    return {k: h.get_tuning(i) for k, i in KNOBS.items()}
+    return {k: h.get_tuning(KNOBS[k]) for k in RUNTIME_KNOBS}
 
 
 def set_schedule(h, sched):
