@@ -659,9 +659,66 @@ __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __r
     }
 }
 
+// Barrier over the lanes that share a row.  When a row fits in one wave
+// (h <= 1024) its lanes execute together and a wave's LDS operations complete
+// in issue order, so a wave-scope fence pair around a compiler-level wave
+// barrier orders the exchange; wider rows need the block barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool WAVE>
+__device__ __forceinline__ void row_sync() {
+    if constexpr (WAVE) wave_sync();
+    else __syncthreads();
+}
+
+// r2c_post / c2r_pre of one row by the T16 lanes that own it
+__device__ __forceinline__ void r2c_post_row(float2* row, int h, const float2* __restrict__ twN, int li, int nl) {
+    for (int k = li; k <= h / 2; k += nl) {
+        if (k == 0) {
+            float2 z = row[0];
+            row[0] = make_float2(z.x + z.y, z.x - z.y);
+        } else {
+            int kk = h - k;
+            float2 A = row[lidx(k)], B = row[lidx(kk)];
+            float2 Bc = cconj(B);
+            float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
+            float2 D = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
+            float2 O = make_float2(D.y, -D.x);
+            float2 WO = cmul(twN[k], O);
+            row[lidx(k)] = cadd(E, WO);
+            if (kk != k) row[lidx(kk)] = cconj(csub(E, WO));
+        }
+    }
+}
+__device__ __forceinline__ void c2r_pre_row(float2* row, int h, const float2* __restrict__ twN, int li, int nl) {
+    for (int k = li; k <= h / 2; k += nl) {
+        if (k == 0) {
+            float2 p = row[0];
+            row[0] = make_float2(p.x + p.y, p.x - p.y);
+        } else {
+            int kk = h - k;
+            float2 P = row[lidx(k)], Q = row[lidx(kk)];
+            float2 S = cadd(P, cconj(Q));
+            float2 D = csub(P, cconj(Q));
+            float2 w = twN[k];
+            float2 iwc = make_float2(w.y, w.x);
+            row[lidx(k)] = cadd(S, cmul(iwc, D));
+            if (kk != k) {
+                float2 S2 = cconj(S);
+                float2 D2 = cconj(make_float2(-D.x, -D.y));
+                float2 imw = make_float2(w.y, -w.x);
+                row[lidx(kk)] = cadd(S2, cmul(imw, D2));
+            }
+        }
+    }
+}
+
 // Row passes for power-of-two h = 16 * MID * 16: T16 = h/16 threads per row,
 // R rows per block, one radix-16 butterfly per thread per stage.
-template <int MID>
+template <int MID, bool WAVE>
 __device__ __forceinline__ void row_mid_stage(float2* rowbuf, int dir, const float2* __restrict__ tw, int li) {
     // radix-MID stage (Ns = 16) of one row held at rowbuf (lidx-padded)
     constexpr int H = 256 * MID;
@@ -674,7 +731,7 @@ __device__ __forceinline__ void row_mid_stage(float2* rowbuf, int dir, const flo
 #pragma unroll
         for (int q = 0; q < MID; ++q) v[b][q] = rowbuf[lidx(j + q * nbf)];
     }
-    __syncthreads();
+    row_sync<WAVE>();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         int j = li * NB + b;
@@ -693,7 +750,7 @@ __device__ __forceinline__ void row_mid_stage(float2* rowbuf, int dir, const flo
 #pragma unroll
         for (int q = 0; q < MID; ++q) rowbuf[lidx(dst + q * 16)] = v[b][q];
     }
-    __syncthreads();
+    row_sync<WAVE>();
 }
 
 template <int MID, bool XTRA>
@@ -701,11 +758,12 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
                                                      Src src, const int* __restrict__ xrow,
                                                      const float4* __restrict__ xent, int cubic) {
     extern __shared__ float2 smc[];
+    constexpr int H = 256 * MID, T16 = 16 * MID;
+    constexpr bool WAVE = T16 <= 64;   // a row's lanes lie in one wave
+    const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
     // row stride in LDS slots: the padded row plus 32/R slots.  The transposed
     // accesses rr * RS + lidx(f) of a half-wave cover R rows x 32/R columns,
     // which then land on 32 distinct 8-byte bank pairs
-    constexpr int H = 256 * MID, T16 = 16 * MID;
-    const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
     const int RS = H + H / 16 + 32 / R;
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
     // logical order: neighbouring row blocks (which share the 128-byte lines
@@ -716,6 +774,8 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
     const int z0 = ((lb / nrb) % nzg) * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     const int2 lr = range ? range[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
+    // extra samples of this block's rows (block-uniform)
+    const bool has_x = XTRA && xrow[span * (Nr + 1) + row0] != xrow[span * (Nr + 1) + row0 + R];
     float2* rowbuf = smc + r * RS;
     for (int z = z0; z < z1; ++z) {
         const float* Fr = F + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
@@ -731,7 +791,7 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
             }
             v[q] = x;
         }
-        if (XTRA) {
+        if (has_x) {
             // rare extra samples sharing a point (adj's rho-wrapped points):
             // add through LDS before the FFT
 #pragma unroll
@@ -750,22 +810,23 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
-            __syncthreads();
+            row_sync<WAVE>();
         }
         dft16(v, -1);
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li * 16 + q)] = v[q];
-        __syncthreads();
-        if (MID > 1) row_mid_stage<MID>(rowbuf, -1, rp.twh, li);
+        row_sync<WAVE>();
+        if (MID > 1) row_mid_stage<MID, WAVE>(rowbuf, -1, rp.twh, li);
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
         twiddle16(v, rp.twh, li, -1);
         dft16(v, -1);
-        __syncthreads();
+        row_sync<WAVE>();
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
-        __syncthreads();
-        r2c_post_at(smc, R, H, rp.twN, [=](int b, int k) { return b * RS + lidx(k); });
+        row_sync<WAVE>();
+        r2c_post_row(rowbuf, H, rp.twN, li, T16);
+        __syncthreads();   // the transposed store reads every row
         float2* out = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
         for (int qq = threadIdx.x; qq < R * H; qq += blockDim.x) {
             int f = qq / R, rr = qq - f * R;
@@ -779,11 +840,10 @@ template <int MID>
 __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restrict__ G) {
     extern __shared__ float2 smc[];
     constexpr int H = 256 * MID, T16 = 16 * MID;
+    constexpr bool WAVE = T16 <= 64;
     const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
     const int RS = H + H / 16 + 32 / R;
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
-    // logical order: neighbouring row blocks (which share the 128-byte lines
-    // of the transposed spectra) run behind the same L2
     const int nrb = Nr / R, nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int row0 = (lb % nrb) * R, span = lb / (nrb * nzg);
@@ -795,17 +855,18 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
         stage_in(
             R * H, [&](int qq) { int f = qq / R, rr = qq - f * R; return in[(size_t)f * Nr + rr]; },
             [&](int qq, float2 v) { int f = qq / R, rr = qq - f * R; smc[rr * RS + lidx(f)] = v; });
-        __syncthreads();
-        c2r_pre_at(smc, R, H, rp.twN, [=](int b, int k) { return b * RS + lidx(k); });
+        __syncthreads();   // every row was written by the whole block
+        c2r_pre_row(rowbuf, H, rp.twN, li, T16);
+        row_sync<WAVE>();
         float2 v[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
         dft16(v, +1);
-        __syncthreads();
+        row_sync<WAVE>();
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li * 16 + q)] = v[q];
-        __syncthreads();
-        if (MID > 1) row_mid_stage<MID>(rowbuf, +1, rp.twh, li);
+        row_sync<WAVE>();
+        if (MID > 1) row_mid_stage<MID, WAVE>(rowbuf, +1, rp.twh, li);
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
         twiddle16(v, rp.twh, li, +1);
@@ -813,7 +874,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
         float* out = G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
 #pragma unroll
         for (int q = 0; q < 16; ++q) *reinterpret_cast<float2*>(out + 2 * (li + q * T16)) = v[q];
-        __syncthreads();
+        __syncthreads();   // the next slice's staging overwrites every row
     }
 }
 
