@@ -77,6 +77,8 @@ struct lp_handle {
     float2* d_fzF = nullptr;
     int2* d_rangeF = nullptr;
     int2* d_rangeA = nullptr;
+    int2* d_wrangeF = nullptr;   // theta columns of G each (span, row) the polar gather reads
+    int2* d_wrangeA = nullptr;   // ... and the Cartesian gather reads
     // LDS-tiled gathers: tile lists and the largest source box of each
     lpk::GTile* d_tilesF = nullptr;   // image -> LP (per span)
     lpk::GTile* d_tilesP = nullptr;   // LP -> polar (per span)
@@ -89,8 +91,9 @@ struct lp_handle {
 
     float* d_Rc = nullptr;
     float2* d_cmapA = nullptr;
-    int* d_xrow = nullptr;
-    float4* d_xent = nullptr;
+    int* d_xrow = nullptr;      // [3][Nrho+1] offsets of each (span, row)'s extra samples in d_xent
+    float4* d_xent = nullptr;   // (theta, x, y, row) of the extra samples
+    int nxent = 0;
     float2* d_cmapC = nullptr;
     float2* d_fzA = nullptr;
     float* d_H2 = nullptr;
@@ -241,6 +244,32 @@ std::vector<lpk::GTile> tile_grid(Coord coord, const std::vector<int>& spans, in
         for (int c = 0; c < ncols; c += TC)
             add_tiles(out, coord, spans, r, c, std::min(TR, re - r), std::min(TC, ncols - c), cap, keep, maxbox);
     return out;
+}
+
+// [lo, hi) theta columns of each (span, row) of the log-polar grids that the
+// source boxes of a gather's tiles cover; a row whose boxes wrap around the
+// periodic theta axis gets the whole row
+std::vector<int2> box_row_ranges(const std::vector<lpk::GTile>& tiles, int Nr, int Nt) {
+    std::vector<int2> r(3 * (size_t)Nr, make_int2(INT_MAX, INT_MIN));
+    std::vector<char> full(3 * (size_t)Nr, 0);
+    for (const auto& t : tiles) {
+        if (t.bw <= 0) continue;
+        bool wraps = t.x0 < 0 || t.x0 + t.bw > Nt;
+        int ny = std::min(t.bh, Nr);
+        for (int yy = 0; yy < ny; ++yy) {
+            int row = ((t.y0 + yy) % Nr + Nr) % Nr;
+            size_t k = (size_t)t.span * Nr + row;
+            if (wraps) full[k] = 1;
+            r[k].x = std::min(r[k].x, t.x0);
+            r[k].y = std::max(r[k].y, t.x0 + t.bw);
+        }
+        if (t.bh > Nr) for (int row = 0; row < Nr; ++row) full[(size_t)t.span * Nr + row] = 1;
+    }
+    for (size_t k = 0; k < r.size(); ++k) {
+        if (full[k]) r[k] = make_int2(0, Nt);
+        else if (r[k].x > r[k].y) r[k] = make_int2(0, 0);
+    }
+    return r;
 }
 
 // [lo, hi) theta bounds of the non-empty points of each (span, row) of a dense map
@@ -446,7 +475,9 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
     }
     h->ntilesF = (int)tF.size();
     h->ntilesP = (int)tP.size();
+    std::vector<int2> wrng = box_row_ranges(tP, Nr, Nt);
     if ((rc = h->upload(&h->d_tilesF, tF)) || (rc = h->upload(&h->d_tilesP, tP))) return rc;
+    if ((rc = h->upload(&h->d_wrangeF, wrng))) return rc;
     if ((rc = h->upload(&h->d_cmapF, cmap)) || (rc = h->upload(&h->d_rangeF, rng)) || (rc = h->upload(&h->d_p2lp, p2lp)) ||
         (rc = h->upload(&h->d_spanrow, spanrow)) || (rc = h->upload(&h->d_fzF, fzT)) ||
         (rc = h->alloc(&h->d_fc, (size_t)h->Ns * N * N)))
@@ -490,7 +521,7 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         if (id < 0 || (size_t)id >= npts) return h->fail(LP_ERR_ARG, "lpids index out of range");
         for (int k = 0; k < 3; ++k) {
             if (!used[id]) cmap[k * npts + id] = make_float2(L2[k][t], L1[k][t]);
-            else extra[k * Nr + id / Nt].push_back(make_float4((float)(id % Nt), L2[k][t], L1[k][t], 0.f));
+            else extra[k * Nr + id / Nt].push_back(make_float4((float)(id % Nt), L2[k][t], L1[k][t], (float)(id / Nt)));
         }
         used[id] = 1;
     }
@@ -499,7 +530,7 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         if (id < 0 || (size_t)id >= npts) return h->fail(LP_ERR_ARG, "wids index out of range");
         for (int k = 0; k < 3; ++k) {
             if (!used[id]) cmap[k * npts + id] = make_float2(W2[k][t], W1[k][t]);
-            else extra[k * Nr + id / Nt].push_back(make_float4((float)(id % Nt), W2[k][t], W1[k][t], 0.f));
+            else extra[k * Nr + id / Nt].push_back(make_float4((float)(id % Nt), W2[k][t], W1[k][t], (float)(id / Nt)));
         }
         used[id] = 1;
     }
@@ -555,11 +586,14 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
     }
     h->ntilesA = (int)tA.size();
     h->ntilesC = (int)tC.size();
+    std::vector<int2> wrng = box_row_ranges(tC, Nr, Nt);
+    if ((rc = h->upload(&h->d_wrangeA, wrng))) return rc;
     if ((rc = h->upload(&h->d_tilesA, tA)) || (rc = h->upload(&h->d_tilesC, tC))) return rc;
     if ((rc = h->upload(&h->d_cmapA, cmap)) || (rc = h->upload(&h->d_rangeA, rng)) || (rc = h->upload(&h->d_cmapC, cmapC)) ||
         (rc = h->upload(&h->d_fzA, fzT)) || (rc = h->alloc(&h->d_Rc, (size_t)h->Ns * h->P * N)))
         return rc;
     if (any_extra && ((rc = h->upload(&h->d_xrow, xrow)) || (rc = h->upload(&h->d_xent, xent)))) return rc;
+    h->nxent = (int)xent.size();
     if (filt) {
         // kernel of the reference filter: h4 = IFFT_4N(ifftshift(filter)) / 4N; keep lags |l| < N
         int M4 = 4 * N, M2 = 2 * N;
@@ -627,13 +661,14 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     }
     mark(3);
     if (h->stop_after == 3) return check_launch(h, "lp_fwd launch");
-    lpk::launch_rows_fwd(rp, h->d_G, h->d_rangeF, src, nullptr, nullptr, h->cubic, s);
+    lpk::launch_rows_fwd(rp, h->d_G, h->d_rangeF, s);
     mark(4);
     if (h->stop_after == 4) return check_launch(h, "lp_fwd launch");
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
     mark(5);
     if (h->stop_after == 5) return check_launch(h, "lp_fwd launch");
-    lpk::launch_rows_inv(rp, h->d_G, s);
+    // only the columns the polar gather reads (all of them when debugging stage 6)
+    lpk::launch_rows_inv(rp, h->d_G, h->stop_after ? nullptr : h->d_wrangeF, s);
     mark(6);
     if (h->stop_after == 6) return check_launch(h, "lp_fwd launch");
     {   // log-polar -> polar for the kept angles, cropped detector
@@ -684,16 +719,18 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
         g.out = h->d_G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxA;
         lpk::launch_gather(g, h->d_tilesA, h->ntilesA, 1, s);
+        // samples that share a log-polar point with another one
+        lpk::launch_add_extras(h->d_G, ns0, h->Nt, h->Nr, src, h->d_xrow, h->d_xent, h->nxent, h->cubic, s);
     }
     mark(3);
     if (h->stop_after == 3) return check_launch(h, "lp_adj launch");
-    lpk::launch_rows_fwd(rp, h->d_G, h->d_rangeA, src, h->d_xrow, h->d_xent, h->cubic, s);
+    lpk::launch_rows_fwd(rp, h->d_G, h->d_rangeA, s);
     mark(4);
     if (h->stop_after == 4) return check_launch(h, "lp_adj launch");
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
     mark(5);
     if (h->stop_after == 5) return check_launch(h, "lp_adj launch");
-    lpk::launch_rows_inv(rp, h->d_G, s);
+    lpk::launch_rows_inv(rp, h->d_G, h->stop_after ? nullptr : h->d_wrangeA, s);
     mark(6);
     if (h->stop_after == 6) return check_launch(h, "lp_adj launch");
     {   // log-polar -> Cartesian, three spans summed, cropped
@@ -735,9 +772,9 @@ int lp_fwd_T(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     lpk::RowPass rp = row_pass(h, ns0);
     rp.conj = 1;
     lpk::Src src{h->d_fc, N, N, (long long)N * N};
-    lpk::launch_rows_fwd(rp, h->d_G, nullptr, src, nullptr, nullptr, h->cubic, s);
+    lpk::launch_rows_fwd(rp, h->d_G, nullptr, s);
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
-    lpk::launch_rows_inv(rp, h->d_G, s);
+    lpk::launch_rows_inv(rp, h->d_G, h->d_rangeF, s);   // the scatter reads the sampled columns only
     float* acc = h->d_fc;   // never aliases G, which the scatter reads
     lpk::launch_zero(acc, (size_t)ns0 * N * N, s);
     {   // transpose of the image -> log-polar gather (x e^rho)
@@ -783,9 +820,9 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     lpk::RowPass rp = row_pass(h, ns0);
     rp.conj = 1;
     lpk::Src src{h->d_Rc, N, h->P, (long long)h->P * N};
-    lpk::launch_rows_fwd(rp, h->d_G, nullptr, src, nullptr, nullptr, h->cubic, s);
+    lpk::launch_rows_fwd(rp, h->d_G, nullptr, s);
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
-    lpk::launch_rows_inv(rp, h->d_G, s);
+    lpk::launch_rows_inv(rp, h->d_G, h->d_rangeA, s);
     float* acc = h->d_Rc;
     lpk::launch_zero(acc, (size_t)ns0 * h->P * N, s);
     {   // transpose of the sinogram -> log-polar gather, plus its extra samples

@@ -182,8 +182,9 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
     }
     const int a = ai / sp.os;
     const float* src = R + ((size_t)z * sp.P0 + a) * sp.N0;
-    float* line = reinterpret_cast<float*>(smc + lds_slots(N));   // N floats
-    float* tmp = line + N;                                        // N floats
+    // N + N floats after the FFT buffer (which is only allocated with a filter)
+    float* line = sp.H2 ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
+    float* tmp = line + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) {
         int xs = min(max(x - sp.lb, 0), sp.N0 - 1);
         line[x] = src[xs];
@@ -276,8 +277,7 @@ __global__ void k_col_iir(const float* __restrict__ in, float* __restrict__ outp
 // gathered samples F (interp + mulexp of lpRgpu.cu:263-267 / 307-311, done by
 // k_gather), real FFT along theta, store the half spectra transposed into S.
 template <int EPT>
-__global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* __restrict__ range, Src src,
-                           const int* __restrict__ xrow, const float4* __restrict__ xent, int cubic, int ldm) {
+__global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* __restrict__ range, int ldm) {
     extern __shared__ float2 smc[];
     const int Nt = rp.Ntheta, h = rp.h, R = rp.R, Nr = rp.Nrho;
     const int row0 = blockIdx.x * R, span = blockIdx.y;
@@ -304,19 +304,6 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
                 int r = p / h, e = p - r * h;
                 smc[lidx(r * ldm + e)] = v;
             });
-        if (xrow) {
-            __syncthreads();
-            const float* T = src.p + (size_t)z * src.zstride;
-            for (int r = 0; r < R; ++r) {
-                int lo = xrow[span * (Nr + 1) + row0 + r], hi = xrow[span * (Nr + 1) + row0 + r + 1];
-                for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-                    float4 q = xent[e];
-                    float v = sample(T, src.W, src.H, q.y, q.z, cubic);
-                    int th = (int)q.x;
-                    atomicAdd(reinterpret_cast<float*>(smc + lidx(r * ldm + (th >> 1))) + (th & 1), v);
-                }
-            }
-        }
         __syncthreads();
         fft_lds<EPT>(smc, ldm, R, rp.planh, rp.twh, -1);
         r2c_post(smc, ldm, R, h, rp.twN);
@@ -456,7 +443,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
 
 // Row pass, inverse direction: C2R along theta for R rows, writing G.
 template <int EPT>
-__global__ void k_rows_inv(RowPass rp, float* __restrict__ G, int ldm) {
+__global__ void k_rows_inv(RowPass rp, float* __restrict__ G, const int2* __restrict__ wrange, int ldm) {
     extern __shared__ float2 smc[];
     const int Nt = rp.Ntheta, h = rp.h, R = rp.R, Nr = rp.Nrho;
     const int row0 = blockIdx.x * R, span = blockIdx.y;
@@ -480,7 +467,9 @@ __global__ void k_rows_inv(RowPass rp, float* __restrict__ G, int ldm) {
         float* out = G + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
         for (int p = threadIdx.x; p < R * h; p += blockDim.x) {
             int r = p / h, e = p - r * h;
-            *reinterpret_cast<float2*>(out + (size_t)r * Nt + 2 * e) = smc[lidx(r * ldm + e)];
+            int2 wr = wrange ? wrange[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
+            if (2 * e + 1 >= wr.x && 2 * e < wr.y)
+                *reinterpret_cast<float2*>(out + (size_t)r * Nt + 2 * e) = smc[lidx(r * ldm + e)];
         }
         __syncthreads();
     }
@@ -753,10 +742,9 @@ __device__ __forceinline__ void row_mid_stage(float2* rowbuf, int dir, const flo
     row_sync<WAVE>();
 }
 
-template <int MID, bool XTRA>
-__global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __restrict__ F, const int2* __restrict__ range,
-                                                     Src src, const int* __restrict__ xrow,
-                                                     const float4* __restrict__ xent, int cubic) {
+template <int MID>
+__global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __restrict__ F,
+                                                     const int2* __restrict__ range) {
     extern __shared__ float2 smc[];
     constexpr int H = 256 * MID, T16 = 16 * MID;
     constexpr bool WAVE = T16 <= 64;   // a row's lanes lie in one wave
@@ -774,8 +762,6 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
     const int z0 = ((lb / nrb) % nzg) * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     const int2 lr = range ? range[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
-    // extra samples of this block's rows (block-uniform)
-    const bool has_x = XTRA && xrow[span * (Nr + 1) + row0] != xrow[span * (Nr + 1) + row0 + R];
     float2* rowbuf = smc + r * RS;
     for (int z = z0; z < z1; ++z) {
         const float* Fr = F + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
@@ -790,27 +776,6 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
                 if (th + 1 >= lr.y) x.y = 0.f;
             }
             v[q] = x;
-        }
-        if (has_x) {
-            // rare extra samples sharing a point (adj's rho-wrapped points):
-            // add through LDS before the FFT
-#pragma unroll
-            for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
-            __syncthreads();
-            const float* T = src.p + (size_t)z * src.zstride;
-            for (int rr = 0; rr < R; ++rr) {
-                int lo = xrow[span * (Nr + 1) + row0 + rr], hi = xrow[span * (Nr + 1) + row0 + rr + 1];
-                for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-                    float4 qd = xent[e];
-                    float sv = sample(T, src.W, src.H, qd.y, qd.z, cubic);
-                    int th = (int)qd.x;
-                    atomicAdd(reinterpret_cast<float*>(smc + rr * RS + lidx(th >> 1)) + (th & 1), sv);
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
-            row_sync<WAVE>();
         }
         dft16(v, -1);
 #pragma unroll
@@ -837,7 +802,8 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
 }
 
 template <int MID>
-__global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restrict__ G) {
+__global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restrict__ G,
+                                                     const int2* __restrict__ wrange) {
     extern __shared__ float2 smc[];
     constexpr int H = 256 * MID, T16 = 16 * MID;
     constexpr bool WAVE = T16 <= 64;
@@ -849,6 +815,10 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
     const int row0 = (lb % nrb) * R, span = lb / (nrb * nzg);
     const int z0 = ((lb / nrb) % nzg) * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
+    // theta columns of this row that the output gather reads (null: all), as
+    // a range of float2 slots
+    const int2 wr = wrange ? wrange[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
+    const int e0 = max(wr.x, 0) >> 1, e1 = min((wr.y + 1) >> 1, H);
     float2* rowbuf = smc + r * RS;
     for (int z = z0; z < z1; ++z) {
         const float2* in = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
@@ -871,9 +841,13 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
         for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
         twiddle16(v, rp.twh, li, +1);
         dft16(v, +1);
-        float* out = G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
+        // back through the row's LDS slots, then only the columns read later
+        row_sync<WAVE>();
 #pragma unroll
-        for (int q = 0; q < 16; ++q) *reinterpret_cast<float2*>(out + 2 * (li + q * T16)) = v[q];
+        for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
+        row_sync<WAVE>();
+        float2* out = reinterpret_cast<float2*>(G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt));
+        for (int e = e0 + li; e < e1; e += T16) out[e] = rowbuf[lidx(e)];
         __syncthreads();   // the next slice's staging overwrites every row
     }
 }
@@ -927,7 +901,8 @@ void launch_image_rows(const float* f, float* fc, int nz, int N0, int N, bool cu
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
     int nt = threads_for(sp.N);
     int ept = ept_for(sp.N, nt);
-    LP_DISPATCH_EPT(ept, k_sino_rows, dim3(sp.P, nz), dim3(nt), sino_lds_bytes(sp.N), s, R, Rc, sp, (int)cubic);
+    size_t lds = sp.H2 ? sino_lds_bytes(sp.N) : 2 * (size_t)sp.N * sizeof(float);
+    LP_DISPATCH_EPT(ept, k_sino_rows, dim3(sp.P, nz), dim3(nt), lds, s, R, Rc, sp, (int)cubic);
 }
 
 void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s) {
@@ -976,46 +951,51 @@ static size_t rows16_lds(const RowPass& rp) {
         }                                                                                                        \
     } while (0)
 
-#define LP_DISPATCH_MID2(mid, KER, B, grid, block, lds, stream, ...)                                                        \
-    do {                                                                                                                 \
-        switch (mid) {                                                                                                   \
-            case 1: allow_lds(KER<1, B>, lds); hipLaunchKernelGGL((KER<1, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
-            case 2: allow_lds(KER<2, B>, lds); hipLaunchKernelGGL((KER<2, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
-            case 4: allow_lds(KER<4, B>, lds); hipLaunchKernelGGL((KER<4, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
-            case 8: allow_lds(KER<8, B>, lds); hipLaunchKernelGGL((KER<8, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
-            default: allow_lds(KER<16, B>, lds); hipLaunchKernelGGL((KER<16, B>), grid, block, lds, stream, __VA_ARGS__); break; \
-        }                                                                                                                \
-    } while (0)
 
-void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const Src& src, const int* xrow,
-                     const float4* xent, bool cubic, hipStream_t s) {
+void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, hipStream_t s) {
     if (int mid = rows16_mid(rp)) {
         dim3 grid((rp.Nrho / rp.R) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(rp.R * (rp.h / 16));
-        if (xrow) LP_DISPATCH_MID2(mid, k_rows16_fwd, true, grid, block, rows16_lds(rp), s, rp, F, range, src, xrow, xent, (int)cubic);
-        else LP_DISPATCH_MID2(mid, k_rows16_fwd, false, grid, block, rows16_lds(rp), s, rp, F, range, src, xrow, xent, (int)cubic);
+        LP_DISPATCH_MID(mid, k_rows16_fwd, grid, block, rows16_lds(rp), s, rp, F, range);
         return;
     }
     int nt = threads_for((long long)rp.R * rp.h);
     int ept = ept_for((long long)rp.R * rp.h, nt);
     int ldm = rp.ldm;
     dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
-    LP_DISPATCH_EPT(ept, k_rows_fwd, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, F, range, src, xrow, xent,
-                    (int)cubic, ldm);
+    LP_DISPATCH_EPT(ept, k_rows_fwd, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, F, range, ldm);
 }
 
-void launch_rows_inv(const RowPass& rp, float* G, hipStream_t s) {
+__global__ void k_add_extras(float* __restrict__ F, int nz, int Nt, int Nr, Src src, const int* __restrict__ xrow,
+                             const float4* __restrict__ xent, int nx, int cubic) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (e >= nx) return;
+    const int span = (e >= xrow[Nr + 1]) + (e >= xrow[2 * (Nr + 1)]);
+    const float4 q = xent[e];
+    const float v = sample(src.p + (size_t)z * src.zstride, src.W, src.H, q.y, q.z, cubic);
+    atomicAdd(F + (((size_t)span * nz + z) * Nr + (int)q.w) * (size_t)Nt + (int)q.x, v);
+}
+
+void launch_add_extras(float* F, int nz, int Ntheta, int Nrho, const Src& src, const int* xrow, const float4* xent,
+                       int nx, bool cubic, hipStream_t s) {
+    if (!xrow || nx <= 0) return;
+    hipLaunchKernelGGL(k_add_extras, dim3((nx + 255) / 256, nz), dim3(256), 0, s, F, nz, Ntheta, Nrho, src, xrow, xent,
+                       nx, (int)cubic);
+}
+
+void launch_rows_inv(const RowPass& rp, float* G, const int2* wrange, hipStream_t s) {
     if (int mid = rows16_mid(rp)) {
         dim3 grid((rp.Nrho / rp.R) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(rp.R * (rp.h / 16));
-        LP_DISPATCH_MID(mid, k_rows16_inv, grid, block, rows16_lds(rp), s, rp, G);
+        LP_DISPATCH_MID(mid, k_rows16_inv, grid, block, rows16_lds(rp), s, rp, G, wrange);
         return;
     }
     int nt = threads_for((long long)rp.R * rp.h);
     int ept = ept_for((long long)rp.R * rp.h, nt);
     int ldm = rp.ldm;
     dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
-    LP_DISPATCH_EPT(ept, k_rows_inv, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, G, ldm);
+    LP_DISPATCH_EPT(ept, k_rows_inv, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, G, wrange, ldm);
 }
 
 // middle radix of M = 16 * MID * 16, or 0 when the register kernel does not apply
@@ -1192,7 +1172,7 @@ __global__ void k_sino_rows_T(const float* __restrict__ Rc, float* __restrict__ 
     const int a = blockIdx.x, z = blockIdx.y;
     const int ai = a * sp.os;
     const float* src = Rc + ((size_t)z * sp.P + ai) * N;
-    float* line = reinterpret_cast<float*>(smc + lds_slots(N));
+    float* line = sp.H2 ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
     float* tmp = line + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) line[x] = src[x];
     __syncthreads();
@@ -1406,7 +1386,8 @@ void launch_image_rows_T(const float* fc, float* f, int nz, int N0, int N, bool 
 void launch_sino_rows_T(const float* Rc, float* R, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
     int nt = threads_for(sp.N);
     int ept = ept_for(sp.N, nt);
-    LP_DISPATCH_EPT(ept, k_sino_rows_T, dim3(sp.P0, nz), dim3(nt), sino_lds_bytes(sp.N), s, Rc, R, sp, (int)cubic);
+    size_t lds = sp.H2 ? sino_lds_bytes(sp.N) : 2 * (size_t)sp.N * sizeof(float);
+    LP_DISPATCH_EPT(ept, k_sino_rows_T, dim3(sp.P0, nz), dim3(nt), lds, s, Rc, R, sp, (int)cubic);
 }
 
 }  // namespace lpk

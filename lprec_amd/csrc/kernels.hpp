@@ -49,12 +49,16 @@ struct RowPass {
 };
 // R2C along theta of the gathered log-polar grids F [3][nz][Nrho][Ntheta];
 // range [3][Nrho]: [lo, hi) theta bounds of the non-empty points of each row
-// (F is read only inside them, zero elsewhere; null = whole rows); xrow/xent: extra samples that
-// share a log-polar point with another one (added on top)
-void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, const Src& src, const int* xrow,
-                     const float4* xent, bool cubic, hipStream_t s);
-// C2R along theta -> G [3][nz][Nrho][Ntheta]
-void launch_rows_inv(const RowPass& rp, float* G, hipStream_t s);
+// (F is read only inside them, zero elsewhere; null = whole rows)
+void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, hipStream_t s);
+// F[span][z][row][theta] += sample(src_z, x, y) for the nx extra samples
+// (theta, x, y, row) of xent (those sharing a log-polar point with another
+// sample); xrow[span * (Nrho + 1)] is the first entry of each span
+void launch_add_extras(float* F, int nz, int Ntheta, int Nrho, const Src& src, const int* xrow, const float4* xent,
+                       int nx, bool cubic, hipStream_t s);
+// C2R along theta -> G [3][nz][Nrho][Ntheta]; only the theta columns
+// [lo, hi) of wrange [3][Nrho] are stored (null: whole rows)
+void launch_rows_inv(const RowPass& rp, float* G, const int2* wrange, hipStream_t s);
 // FFT along rho, x fZ, inverse, in place on S; fzT [h+1][Nrho] (scaled)
 void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s);
 
