@@ -170,7 +170,7 @@ __global__ void k_image_rows(const float* __restrict__ f, float* __restrict__ fc
 // circular convolution of the 4N zero-padded row with a real even kernel; only lags
 // |l| < N reach the N outputs, so it is evaluated exactly as a length-2N circular
 // convolution with the kernel restricted to those lags (spectrum H2, built on host).
-template <int EPT>
+template <int EPT, bool FILT>
 __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc, SinoPrep sp, int cubic) {
     extern __shared__ float2 smc[];
     const int N = sp.N;
@@ -183,13 +183,13 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
     const int a = ai / sp.os;
     const float* src = R + ((size_t)z * sp.P0 + a) * sp.N0;
     // N + N floats after the FFT buffer (which is only allocated with a filter)
-    float* line = sp.H2 ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
+    float* line = FILT ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
     float* tmp = line + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) {
         int xs = min(max(x - sp.lb, 0), sp.N0 - 1);
         line[x] = src[xs];
     }
-    if (sp.H2) {
+    if (FILT) {
         __syncthreads();
         // length-2N real signal [line, 0] as N complex samples
         for (int e = threadIdx.x; e < N; e += blockDim.x)
@@ -328,8 +328,8 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // Stage the box [x0, x0+bw) x [y0, y0+bh) of the periodic grid S [H][W] into
 // LDS rows of stride bw|1, 8 loads in flight per lane, 32-bit offsets from
 // the block-uniform base.
+template <int UL>
 __device__ __forceinline__ void load_box(float* box, const float* __restrict__ S, const GTile& t, int W, int H) {
-    constexpr int UL = 8;
     const int nt = blockDim.x;
     const int nbox = t.bw * t.bh;
     const int bs = t.bw | 1;
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
 #pragma unroll 1
         for (int gi = 0; gi < GROUP; ++gi) {
             const GTile t = tb[gi];
-            if (t.bw > 0) load_box(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
+            if (t.bw > 0) load_box<8>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
             __syncthreads();
             const float2* cbase = g.coord + (size_t)t.span * g.c_span;
 #pragma unroll 2
@@ -516,6 +516,30 @@ __global__ void k_cols(RowPass rp, FftPlan planM, const float2* __restrict__ twM
     }
 }
 
+// w^q (q = 1..15) applied to v[q] from table reads of w, w^2, w^4, w^8 and at
+// most three products each
+__device__ __forceinline__ void twiddle16(float2* v, const float2* __restrict__ tw, int base, int dir) {
+    float2 w1 = twiddle(tw, base, dir), w2 = twiddle(tw, 2 * base, dir);
+    float2 w4 = twiddle(tw, 4 * base, dir), w8 = twiddle(tw, 8 * base, dir);
+    float2 w3 = cmul(w1, w2);
+    v[1] = cmul(v[1], w1);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], cmul(w1, w4));
+    v[6] = cmul(v[6], cmul(w2, w4));
+    v[7] = cmul(v[7], cmul(w3, w4));
+    v[8] = cmul(v[8], w8);
+    v[9] = cmul(v[9], cmul(w1, w8));
+    v[10] = cmul(v[10], cmul(w2, w8));
+    v[11] = cmul(v[11], cmul(w3, w8));
+    float2 w12 = cmul(w4, w8);
+    v[12] = cmul(v[12], w12);
+    v[13] = cmul(v[13], cmul(w1, w12));
+    v[14] = cmul(v[14], cmul(w2, w12));
+    v[15] = cmul(v[15], cmul(w3, w12));
+}
+
 // Column pass for power-of-two M = 16 * MID * 16 (MID in 1, 2, 4, 8, 16):
 // M/16 threads per column hold one radix-16 butterfly each.  Stage 0 reads
 // HBM directly, the last forward stage leaves thread i holding outputs
@@ -541,7 +565,9 @@ __device__ __forceinline__ void col_mid_stage(float2* buf, int M, int dir, const
         int j = li * NB + b;
         int k = j & 15;
         const int tstep = M / (16 * MID);
-        if (MID > 1) {
+        if (MID == 16) {
+            twiddle16(v[b], tw, k * tstep, dir);   // powers formed as they are applied
+        } else if (MID > 1) {
             float2 w[MID > 1 ? MID : 2];
             twiddle_pows<MID>(tw, k * tstep, dir, w);
 #pragma unroll
@@ -560,29 +586,6 @@ __device__ __forceinline__ void col_mid_stage(float2* buf, int M, int dir, const
     __syncthreads();
 }
 
-// w^q (q = 1..15) applied to v[q] from table reads of w, w^2, w^4, w^8 and at
-// most three products each
-__device__ __forceinline__ void twiddle16(float2* v, const float2* __restrict__ tw, int base, int dir) {
-    float2 w1 = twiddle(tw, base, dir), w2 = twiddle(tw, 2 * base, dir);
-    float2 w4 = twiddle(tw, 4 * base, dir), w8 = twiddle(tw, 8 * base, dir);
-    float2 w3 = cmul(w1, w2);
-    v[1] = cmul(v[1], w1);
-    v[2] = cmul(v[2], w2);
-    v[3] = cmul(v[3], w3);
-    v[4] = cmul(v[4], w4);
-    v[5] = cmul(v[5], cmul(w1, w4));
-    v[6] = cmul(v[6], cmul(w2, w4));
-    v[7] = cmul(v[7], cmul(w3, w4));
-    v[8] = cmul(v[8], w8);
-    v[9] = cmul(v[9], cmul(w1, w8));
-    v[10] = cmul(v[10], cmul(w2, w8));
-    v[11] = cmul(v[11], cmul(w3, w8));
-    float2 w12 = cmul(w4, w8);
-    v[12] = cmul(v[12], w12);
-    v[13] = cmul(v[13], cmul(w1, w12));
-    v[14] = cmul(v[14], cmul(w2, w12));
-    v[15] = cmul(v[15], cmul(w3, w12));
-}
 
 template <int MID>
 __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __restrict__ twM,
@@ -725,7 +728,9 @@ __device__ __forceinline__ void row_mid_stage(float2* rowbuf, int dir, const flo
     for (int b = 0; b < NB; ++b) {
         int j = li * NB + b;
         int k = j & 15;
-        if (MID > 1) {
+        if (MID == 16) {
+            twiddle16(v[b], tw, k * (H / (16 * MID)), dir);   // powers formed as they are applied
+        } else if (MID > 1) {
             float2 w[MID > 1 ? MID : 2];
             twiddle_pows<MID>(tw, k * (H / (16 * MID)), dir, w);
 #pragma unroll
@@ -884,6 +889,16 @@ static void allow_lds(K k, size_t bytes) {
         }                                                                                         \
     } while (0)
 
+#define LP_DISPATCH_EPT2(ept, KER, B, grid, block, lds, stream, ...)                                                    \
+    do {                                                                                                             \
+        switch (ept) {                                                                                               \
+            case 4: allow_lds(KER<4, B>, lds); hipLaunchKernelGGL((KER<4, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 8: allow_lds(KER<8, B>, lds); hipLaunchKernelGGL((KER<8, B>), grid, block, lds, stream, __VA_ARGS__); break;   \
+            case 16: allow_lds(KER<16, B>, lds); hipLaunchKernelGGL((KER<16, B>), grid, block, lds, stream, __VA_ARGS__); break; \
+            default: allow_lds(KER<32, B>, lds); hipLaunchKernelGGL((KER<32, B>), grid, block, lds, stream, __VA_ARGS__); break; \
+        }                                                                                                            \
+    } while (0)
+
 // threads per FFT block: keep <= 16 points per thread (EPT 32 spills past
 // the 128-VGPR budget of a 1024-thread-capable kernel)
 static int threads_for(long long points) {
@@ -901,8 +916,12 @@ void launch_image_rows(const float* f, float* fc, int nz, int N0, int N, bool cu
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
     int nt = threads_for(sp.N);
     int ept = ept_for(sp.N, nt);
-    size_t lds = sp.H2 ? sino_lds_bytes(sp.N) : 2 * (size_t)sp.N * sizeof(float);
-    LP_DISPATCH_EPT(ept, k_sino_rows, dim3(sp.P, nz), dim3(nt), lds, s, R, Rc, sp, (int)cubic);
+    if (!sp.H2) {
+        hipLaunchKernelGGL((k_sino_rows<4, false>), dim3(sp.P, nz), dim3(256), 2 * (size_t)sp.N * sizeof(float), s, R,
+                           Rc, sp, (int)cubic);
+        return;
+    }
+    LP_DISPATCH_EPT2(ept, k_sino_rows, true, dim3(sp.P, nz), dim3(nt), sino_lds_bytes(sp.N), s, R, Rc, sp, (int)cubic);
 }
 
 void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s) {
@@ -1164,7 +1183,7 @@ __global__ void k_image_rows_T(const float* __restrict__ fc, float* __restrict__
 // transpose, the FBP convolution (a symmetric Toeplitz matrix: its own
 // transpose), then the transpose of the edge replication (the columns left of
 // lb fold into column lb, those right of rb into rb) and of the placement.
-template <int EPT>
+template <int EPT, bool FILT>
 __global__ void k_sino_rows_T(const float* __restrict__ Rc, float* __restrict__ R, SinoPrep sp, int cubic) {
     extern __shared__ float2 smc[];
     __shared__ float red[2][32];
@@ -1172,12 +1191,12 @@ __global__ void k_sino_rows_T(const float* __restrict__ Rc, float* __restrict__ 
     const int a = blockIdx.x, z = blockIdx.y;
     const int ai = a * sp.os;
     const float* src = Rc + ((size_t)z * sp.P + ai) * N;
-    float* line = sp.H2 ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
+    float* line = FILT ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
     float* tmp = line + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) line[x] = src[x];
     __syncthreads();
     if (cubic) iir_line_lds_T(line, tmp, N);
-    if (sp.H2) {
+    if (FILT) {
         for (int e = threadIdx.x; e < N; e += blockDim.x)
             smc[lidx(e)] = 2 * e < N ? make_float2(line[2 * e], line[2 * e + 1]) : make_float2(0.f, 0.f);
         __syncthreads();
@@ -1386,8 +1405,12 @@ void launch_image_rows_T(const float* fc, float* f, int nz, int N0, int N, bool 
 void launch_sino_rows_T(const float* Rc, float* R, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
     int nt = threads_for(sp.N);
     int ept = ept_for(sp.N, nt);
-    size_t lds = sp.H2 ? sino_lds_bytes(sp.N) : 2 * (size_t)sp.N * sizeof(float);
-    LP_DISPATCH_EPT(ept, k_sino_rows_T, dim3(sp.P0, nz), dim3(nt), lds, s, Rc, R, sp, (int)cubic);
+    if (!sp.H2) {
+        hipLaunchKernelGGL((k_sino_rows_T<4, false>), dim3(sp.P0, nz), dim3(256), 2 * (size_t)sp.N * sizeof(float), s,
+                           Rc, R, sp, (int)cubic);
+        return;
+    }
+    LP_DISPATCH_EPT2(ept, k_sino_rows_T, true, dim3(sp.P0, nz), dim3(nt), sino_lds_bytes(sp.N), s, Rc, R, sp, (int)cubic);
 }
 
 }  // namespace lpk
