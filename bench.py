@@ -59,19 +59,21 @@ def pipeline_bytes(g):
     return (4 * N0 * N0 + 16 * N * N + 12 * L + 4 * N0 * P0) + (4 * N0 * P0 + 16 * N * P + 12 * L + 4 * N0 * N0)
 
 
-# Kernel -> stage of the pipeline (one fwd or adj call launches each stage once).
+# Kernel name prefix -> stage of the pipeline (one fwd or adj call launches
+# each stage once; the first matching prefix wins).
 _STAGE_OF_KERNEL = (
     ("k_image_rows", "prep_rows"), ("k_sino_rows", "sino_rows"), ("k_col_iir", "col_iir"),
-    ("k_gather<0>", "gather_lp"), ("k_gather<1>", "gather_lp"), ("k_gather<2>", "polar_out"),
-    ("k_gather<3>", "cart_out"), ("k_rows16_fwd", "rows_fwd"), ("k_rows_fwd", "rows_fwd"),
-    ("k_cols16", "cols"), ("k_cols", "cols"), ("k_rows16_inv", "rows_inv"), ("k_rows_inv", "rows_inv"),
+    ("k_gather<0", "gather_lp"), ("k_gather<1", "gather_lp"), ("k_add_extras", "gather_lp"),
+    ("k_gather<2", "polar_out"), ("k_gather<3", "cart_out"),
+    ("k_rows16_fwd", "rows_fwd"), ("k_rows_fwd", "rows_fwd"), ("k_cols16", "cols"), ("k_cols", "cols"),
+    ("k_rows16_inv", "rows_inv"), ("k_rows_inv", "rows_inv"),
 )
 
 
 def _stage_of(kernel_name):
     base = kernel_name.replace("void ", "").split("(")[0].replace("lpk::", "")
     for key, stage in _STAGE_OF_KERNEL:
-        if base == key or (not key.endswith(">") and base.startswith(key + "<")):
+        if base.startswith(key):
             return stage
     return None
 
@@ -123,20 +125,20 @@ def pmc_summary(root, copy_mib):
 # recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
 MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536900783, 536870912),
-    "fwd.col_iir": (795737614, 536870912),
-    "fwd.gather_lp": (5527993134, 1133248512),
-    "fwd.rows_fwd": (1411006259, 3422552064),
-    "fwd.cols": (4831650994, 3221323776),
-    "fwd.rows_inv": (6445603614, 3221225728),
-    "fwd.polar_out": (5262708756, 393216000),
-    "adj.sino_rows": (393264505, 786432000),
-    "adj.col_iir": (1168524964, 786432000),
-    "adj.gather_lp": (2593658431, 1610686480),
-    "adj.rows_fwd": (1957611386, 3422552064),
-    "adj.cols": (4831645810, 3221323776),
-    "adj.rows_inv": (6445715164, 3221225664),
-    "adj.cart_out": (5066737552, 536870912),
+    "fwd.prep_rows": (536916858, 536870912),
+    "fwd.col_iir": (795739678, 536870912),
+    "fwd.gather_lp": (4494355256, 1133599808),
+    "fwd.rows_fwd": (1097154233, 3350206048),
+    "fwd.cols": (3256788322, 3221422080),
+    "fwd.rows_inv": (3230251711, 1623203008),
+    "fwd.polar_out": (4653016336, 393216000),
+    "adj.sino_rows": (393255296, 786432000),
+    "adj.col_iir": (1168477435, 786432000),
+    "adj.gather_lp": (4088179137, 1610832848),
+    "adj.rows_fwd": (1617652836, 3413262496),
+    "adj.cols": (3256744802, 3221422080),
+    "adj.rows_inv": (3227553884, 1084601456),
+    "adj.cart_out": (3545455366, 536870912),
 }
 
 
