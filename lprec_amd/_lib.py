@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblprec_amd.so")
+# LPREC_AMD_LIB overrides the in-tree library (e.g. a build with other flags)
+LIB_PATH = os.environ.get("LPREC_AMD_LIB") or os.path.join(HERE, "liblprec_amd.so")
 
 LP_OK = 0
 LP_ERR_ARG = -1
