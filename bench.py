@@ -315,10 +315,18 @@ def main():
         n = args.copy_ref * (1 << 20) // 4
         a = torch.ones(n, device="cuda")
         b = torch.empty(n, device="cuda")
+        b.copy_(a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         for _ in range(max(args.steps, 1)):
             b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_ref_MiB": args.copy_ref, "launches": max(args.steps, 1)}))
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / max(args.steps, 1)
+        print(json.dumps({"copy_ref_MiB": args.copy_ref, "launches": max(args.steps, 1) + 1,
+                          "ms_per_copy": round(ms, 4),
+                          "read_plus_write_GBps": round(2 * args.copy_ref * (1 << 20) / (ms * 1e-3) / 1e9, 1)}))
         return
     N0, P0, ns = args.N, args.nproj, args.slices
     cor = N0 / 2
