@@ -215,7 +215,7 @@ void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>
         lpk::GTile& g = t[i];
         g.r0 = r0; g.c0 = c0; g.nr = nr; g.nc = nc; g.span = spans[i]; g.pad = 0;
         tile_box(coord, spans[i], r0, c0, nr, nc, g.x0, g.y0, g.bw, g.bh);
-        if ((long long)(g.bw | 1) * g.bh > cap) fits = false;
+        if ((long long)(g.bw | 1) * (g.bh | 1) > cap) fits = false;   // either box layout fits
     }
     if (!fits) {
         if (nr == 1 && nc == 1) throw std::runtime_error("a single sample exceeds the gather box budget");
@@ -231,7 +231,7 @@ void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>
         return;
     }
     for (auto& g : t) {
-        maxbox = std::max(maxbox, (g.bw | 1) * g.bh);
+        maxbox = std::max(maxbox, (g.bw | 1) * (g.bh | 1));
         out.push_back(g);
     }
 }

@@ -328,11 +328,14 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // Stage the box [x0, x0+bw) x [y0, y0+bh) of the periodic grid S [H][W] into
 // LDS rows of stride bw|1, 8 loads in flight per lane, 32-bit offsets from
 // the block-uniform base.
-template <int UL>
+// COLMAJOR stores the box transposed (column stride bh|1): the gathers whose
+// neighbouring samples step along the source rows (sinogram -> LP, LP ->
+// polar) then read consecutive LDS banks.
+template <int UL, bool COLMAJOR>
 __device__ __forceinline__ void load_box(float* box, const float* __restrict__ S, const GTile& t, int W, int H) {
     const int nt = blockDim.x;
     const int nbox = t.bw * t.bh;
-    const int bs = t.bw | 1;
+    const int bs = t.bw | 1, cs = t.bh | 1;
     // (i + 0.5) / bw in float is exact enough for i, bw < 2^20
     const float inv = 1.0f / (float)t.bw;
     for (int i0 = threadIdx.x; i0 < nbox; i0 += UL * nt) {
@@ -353,20 +356,23 @@ __device__ __forceinline__ void load_box(float* box, const float* __restrict__ S
             int i = i0 + u * nt;
             if (i < nbox) {
                 int yy = (int)(((float)i + 0.5f) * inv);
-                box[yy * bs + i - yy * t.bw] = v[u];
+                int xx = i - yy * t.bw;
+                box[COLMAJOR ? xx * cs + yy : yy * bs + xx] = v[u];
             }
         }
     }
 }
 
 // One sample of the staged box at source coordinate c (valid).
+template <bool COLMAJOR>
 __device__ __forceinline__ float box_sample(const float* box, const GTile& t, float2 c, int cubic) {
     float fx = floorf(c.x), fy = floorf(c.y);
     float ax = c.x - fx, ay = c.y - fy;
-    const int bs = t.bw | 1;
+    // element (x, y) of the box sits at x * sx + y * sy
+    const int sx = COLMAJOR ? (t.bh | 1) : 1, sy = COLMAJOR ? 1 : (t.bw | 1);
     int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
     int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
-    const float* q = box + ly * bs + lx;
+    const float* q = box + ly * sy + lx * sx;
     if (cubic) {
         float wx[4], wy[4];
         bspline_w(ax, wx);
@@ -374,14 +380,14 @@ __device__ __forceinline__ float box_sample(const float* box, const GTile& t, fl
         float v = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float* qr = q + b * bs;
-            v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
+            const float* qr = q + b * sy;
+            v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[sx] + wx[2] * qr[2 * sx] + wx[3] * qr[3 * sx]);
         }
         return v;
     }
-    const float* q1 = q + bs + 1;
-    const float* q2 = q1 + bs;
-    return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
+    const float* q1 = q + sy + sx;
+    const float* q2 = q1 + sy;
+    return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[sx]) + ay * ((1.f - ax) * q2[0] + ax * q2[sx]);
 }
 
 template <int KIND, int GROUP>
@@ -406,7 +412,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
 #pragma unroll 1
         for (int gi = 0; gi < GROUP; ++gi) {
             const GTile t = tb[gi];
-            if (t.bw > 0) load_box<8>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
+            constexpr bool CM = false;   // row-major boxes (a column-major box measured slower for every kind)
+            if (t.bw > 0) load_box<8, CM>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
             __syncthreads();
             const float2* cbase = g.coord + (size_t)t.span * g.c_span;
 #pragma unroll 2
@@ -415,7 +422,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
                 if (p < npts) {
                     int r = p / t.nc, cidx = p - r * t.nc;
                     float2 c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
-                    float v = (c.x == c.x) ? box_sample(box, t, c, g.cubic) : 0.f;
+                    float v = (c.x == c.x) ? box_sample<CM>(box, t, c, g.cubic) : 0.f;
                     if (GROUP == 1) {
                         if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
                         O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + cidx] = v;
