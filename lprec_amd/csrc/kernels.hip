@@ -412,24 +412,54 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
 #pragma unroll 1
         for (int gi = 0; gi < GROUP; ++gi) {
             const GTile t = tb[gi];
-            constexpr bool CM = false;   // row-major boxes (a column-major box measured slower for every kind)
+            constexpr bool CM = false;   // row-major boxes (column-major measured slower for the sinogram gather)
+            // coordinates run two points ahead of the sampling; the first two are
+            // issued before the box load and arrive while it streams in
+            const float2* cbase = g.coord + (size_t)t.span * g.c_span;
+            auto coord_of = [&](int e) {
+                int p = threadIdx.x + e * nt;
+                float2 c = make_float2(__int_as_float(0x7fc00000), 0.f);   // NaN: no sample
+                if (p < npts) {
+                    int r = p / t.nc, cidx = p - r * t.nc;
+                    c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
+                }
+                return c;
+            };
+            float2 ca = coord_of(0), cb = coord_of(1);
             if (t.bw > 0) load_box<8, CM>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
             __syncthreads();
-            const float2* cbase = g.coord + (size_t)t.span * g.c_span;
-#pragma unroll 2
-            for (int e = 0; e < PPT; ++e) {
+            auto sample_point = [&](int e, float2 c) {
                 int p = threadIdx.x + e * nt;
                 if (p < npts) {
                     int r = p / t.nc, cidx = p - r * t.nc;
-                    float2 c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
                     float v = (c.x == c.x) ? box_sample<CM>(box, t, c, g.cubic) : 0.f;
                     if (GROUP == 1) {
                         if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
                         O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + cidx] = v;
                     } else {
-                        acc[e] += v;
+                        return v;
                     }
                 }
+                return 0.f;
+            };
+            static_assert(PPT % 2 == 0, "points per thread come in pairs");
+#pragma unroll 1
+            for (int e = 0; e < PPT; e += 2) {
+                float2 na = ca, nb = cb;
+                if (e + 2 < PPT) {
+                    na = coord_of(e + 2);
+                    nb = coord_of(e + 3);
+                }
+                float va = sample_point(e, ca), vb = sample_point(e + 1, cb);
+                if constexpr (GROUP > 1) {
+                    // acc is indexed by a loop variable: keep it in registers by
+                    // selecting the pair explicitly
+#pragma unroll
+                    for (int q = 0; q < PPT; q += 2)
+                        if (q == e) { acc[q] += va; acc[q + 1] += vb; }
+                }
+                ca = na;
+                cb = nb;
             }
             __syncthreads();
         }
