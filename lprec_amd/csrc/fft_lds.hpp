@@ -216,6 +216,109 @@ __device__ void fft_stage_generic(float2* buf, int ldm, int nlines, int M, int N
     __syncthreads();
 }
 
+// Odd radix R > 5 by conjugate pairs: with x'_q the twiddled inputs,
+// s_q = x'_q + x'_{R-q}, d_q = x'_q - x'_{R-q} (q = 1..(R-1)/2) and
+// w_m = exp(dir 2 pi i m / R),
+//   y_0 = x'_0 + sum s_q,  y_p = A_p + i B_p,  y_{R-p} = A_p - i B_p,
+//   A_p = x'_0 + sum_q s_q Re w_{pq},  B_p = sum_q d_q Im w_{pq}.
+// A work item is one butterfly and a group of up to PP output pairs, so each
+// input is read and twiddled once per group instead of once per output.
+// Returns false (doing nothing) when a thread would need more than MAXW work
+// items; the caller then uses fft_stage_generic.
+template <int EPT>
+__device__ bool fft_stage_oddsym(float2* buf, int ldm, int nlines, int M, int Ns, int R,
+                                 const float2* __restrict__ tw, int dir) {
+    constexpr int PP = 8, MAXW = 1, MAXR = 128;
+    const int nbf = M / R, hR = (R - 1) / 2;
+    const int G = (hR + PP - 1) / PP;
+    const int total = nlines * nbf * G;
+    const int nt = blockDim.x;
+    if (R > MAXR || (R & 1) == 0 || total > MAXW * nt) return false;
+    __shared__ float2 wtab[MAXR];   // w_m, m < R
+    for (int m = threadIdx.x; m < R; m += nt) wtab[m] = twiddle(tw, m * nbf, dir);
+    __syncthreads();
+    const int tstep = M / (Ns * R);
+    float2 ya[MAXW][PP], yb[MAXW][PP], y0[MAXW];
+#pragma unroll
+    for (int c = 0; c < MAXW; ++c) {
+        int t = threadIdx.x + c * nt;
+        if (t < total) {
+            int g = t % G, bi = t / G;
+            int b = bi / nbf, i = bi - b * nbf;
+            int k = i % Ns;
+            const int src = b * ldm + i;
+            const int p0 = 1 + g * PP;
+            const float2 x0 = buf[lidx(src)];
+            float2 A[PP], B[PP];
+            int mq[PP];
+#pragma unroll
+            for (int j = 0; j < PP; ++j) { A[j] = x0; B[j] = make_float2(0.f, 0.f); mq[j] = 0; }
+            float2 S0 = x0;
+            // twiddles W^(q e1) (ascending q) and W^((R-q) e1) (descending), by
+            // recurrence re-read from the table every 8 terms
+            const int e1 = k * tstep;
+            const float2 w1 = twiddle(tw, e1, dir);
+            const float2 w1c = make_float2(w1.x, -w1.y);
+            float2 wa = make_float2(1.f, 0.f), wb = make_float2(1.f, 0.f);
+            for (int q = 1; q <= hR; ++q) {
+                float2 xa = buf[lidx(src + q * nbf)], xb = buf[lidx(src + (R - q) * nbf)];
+                if (Ns > 1) {
+                    if (((q - 1) & 7) == 0) {
+                        wa = twiddle(tw, (int)(((long long)q * e1) % M), dir);
+                        wb = twiddle(tw, (int)(((long long)(R - q) * e1) % M), dir);
+                    }
+                    xa = cmul(xa, wa);
+                    xb = cmul(xb, wb);
+                    wa = cmul(wa, w1);
+                    wb = cmul(wb, w1c);
+                }
+                const float2 sq = cadd(xa, xb), dq = csub(xa, xb);
+                S0 = cadd(S0, sq);
+#pragma unroll
+                for (int j = 0; j < PP; ++j) {
+                    // m = (p q) mod R, advanced by p each step
+                    int m = mq[j] + p0 + j;
+                    m = m >= R ? m - R : m;
+                    mq[j] = m;
+                    const float2 w = wtab[m];
+                    A[j].x += sq.x * w.x; A[j].y += sq.y * w.x;
+                    B[j].x += dq.x * w.y; B[j].y += dq.y * w.y;
+                }
+            }
+            y0[c] = S0;
+#pragma unroll
+            for (int j = 0; j < PP; ++j) {
+                const float2 iB = make_float2(-B[j].y, B[j].x);
+                ya[c][j] = cadd(A[j], iB);
+                yb[c][j] = csub(A[j], iB);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXW; ++c) {
+        int t = threadIdx.x + c * nt;
+        if (t < total) {
+            int g = t % G, bi = t / G;
+            int b = bi / nbf, i = bi - b * nbf;
+            int k = i % Ns;
+            const int dst = b * ldm + (i - k) * R + k;
+            const int p0 = 1 + g * PP;
+            if (g == 0) buf[lidx(dst)] = y0[c];
+#pragma unroll
+            for (int j = 0; j < PP; ++j) {
+                int p = p0 + j;
+                if (p <= hR) {
+                    buf[lidx(dst + p * Ns)] = ya[c][j];
+                    buf[lidx(dst + (R - p) * Ns)] = yb[c][j];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
 // Full in-place FFT (dir = -1 forward, +1 inverse, unnormalised) of nlines
 // lines of length plan.M.  Ends with a barrier.
 template <int EPT>
@@ -230,7 +333,10 @@ __device__ void fft_lds(float2* buf, int ldm, int nlines, const FftPlan& plan, c
             case 4: fft_stage<4, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
             case 5: fft_stage<5, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
             case 8: fft_stage<8, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
-            default: fft_stage_generic<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir); break;
+            default:
+                if (!fft_stage_oddsym<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir))
+                    fft_stage_generic<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir);
+                break;
         }
         Ns *= R;
     }

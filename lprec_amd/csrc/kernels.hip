@@ -328,6 +328,13 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // Stage the box [x0, x0+bw) x [y0, y0+bh) of the periodic grid S [H][W] into
 // LDS rows of stride bw|1, 8 loads in flight per lane, 32-bit offsets from
 // the block-uniform base.
+// Row and column of point p of a tile with nc columns, from a float
+// reciprocal (exact for p, nc < 2^20) instead of an integer division.
+__device__ __forceinline__ void tile_rc(int p, int nc, float inv_nc, int& r, int& c) {
+    r = (int)(((float)p + 0.5f) * inv_nc);
+    c = p - r * nc;
+}
+
 // COLMAJOR stores the box transposed (column stride bh|1): the gathers whose
 // neighbouring samples step along the source rows (sinogram -> LP, LP ->
 // polar) then read consecutive LDS banks.
@@ -413,53 +420,73 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
         for (int gi = 0; gi < GROUP; ++gi) {
             const GTile t = tb[gi];
             constexpr bool CM = false;   // row-major boxes (column-major measured slower for the sinogram gather)
-            // coordinates run two points ahead of the sampling; the first two are
-            // issued before the box load and arrive while it streams in
+            // coordinates (and e^rho scales) run GP points ahead of the sampling;
+            // the first GP are issued before the box load and arrive while it
+            // streams in
+            constexpr int GP = kGatherPipe;
+            static_assert(PPT % GP == 0, "points per thread come in groups of GP");
             const float2* cbase = g.coord + (size_t)t.span * g.c_span;
             auto coord_of = [&](int e) {
                 int p = threadIdx.x + e * nt;
                 float2 c = make_float2(__int_as_float(0x7fc00000), 0.f);   // NaN: no sample
                 if (p < npts) {
-                    int r = p / t.nc, cidx = p - r * t.nc;
+                    int r, cidx;
+                    tile_rc(p, t.nc, 1.0f / (float)t.nc, r, cidx);
                     c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
                 }
                 return c;
             };
-            float2 ca = coord_of(0), cb = coord_of(1);
+            auto scale_of = [&](int e) {
+                int p = threadIdx.x + e * nt;
+                float sc = 1.f;
+                if (GROUP == 1 && g.scale && p < npts) {
+                    int r, cidx;
+                    tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, cidx);
+                    sc = g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
+                }
+                return sc;
+            };
+            float2 cur[GP];
+            float scur[GP];
+#pragma unroll
+            for (int u = 0; u < GP; ++u) { cur[u] = coord_of(u); scur[u] = scale_of(u); }
             if (t.bw > 0) load_box<8, CM>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
             __syncthreads();
-            auto sample_point = [&](int e, float2 c) {
-                int p = threadIdx.x + e * nt;
-                if (p < npts) {
-                    int r = p / t.nc, cidx = p - r * t.nc;
-                    float v = (c.x == c.x) ? box_sample<CM>(box, t, c, g.cubic) : 0.f;
-                    if (GROUP == 1) {
-                        if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
-                        O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + cidx] = v;
-                    } else {
-                        return v;
+#pragma unroll 1
+            for (int e = 0; e < PPT; e += GP) {
+                float2 nxt[GP];
+                float snxt[GP];
+#pragma unroll
+                for (int u = 0; u < GP; ++u) { nxt[u] = cur[u]; snxt[u] = scur[u]; }
+                if (e + GP < PPT) {
+#pragma unroll
+                    for (int u = 0; u < GP; ++u) { nxt[u] = coord_of(e + GP + u); snxt[u] = scale_of(e + GP + u); }
+                }
+                float vals[GP];
+#pragma unroll
+                for (int u = 0; u < GP; ++u) {
+                    int p = threadIdx.x + (e + u) * nt;
+                    const float2 c = cur[u];
+                    float v = (p < npts && c.x == c.x) ? box_sample<CM>(box, t, c, g.cubic) : 0.f;
+                    vals[u] = v;
+                    if (GROUP == 1 && p < npts) {
+                        int r, cidx;
+                        tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, cidx);
+                        O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + cidx] = v * scur[u];
                     }
                 }
-                return 0.f;
-            };
-            static_assert(PPT % 2 == 0, "points per thread come in pairs");
-#pragma unroll 1
-            for (int e = 0; e < PPT; e += 2) {
-                float2 na = ca, nb = cb;
-                if (e + 2 < PPT) {
-                    na = coord_of(e + 2);
-                    nb = coord_of(e + 3);
-                }
-                float va = sample_point(e, ca), vb = sample_point(e + 1, cb);
                 if constexpr (GROUP > 1) {
                     // acc is indexed by a loop variable: keep it in registers by
-                    // selecting the pair explicitly
+                    // selecting the group explicitly
 #pragma unroll
-                    for (int q = 0; q < PPT; q += 2)
-                        if (q == e) { acc[q] += va; acc[q + 1] += vb; }
+                    for (int q = 0; q < PPT; q += GP)
+                        if (q == e) {
+#pragma unroll
+                            for (int u = 0; u < GP; ++u) acc[q + u] += vals[u];
+                        }
                 }
-                ca = na;
-                cb = nb;
+#pragma unroll
+                for (int u = 0; u < GP; ++u) { cur[u] = nxt[u]; scur[u] = snxt[u]; }
             }
             __syncthreads();
         }
@@ -468,7 +495,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
             for (int e = 0; e < PPT; ++e) {
                 int p = threadIdx.x + e * nt;
                 if (p < npts) {
-                    int r = p / t0.nc, c = p - r * t0.nc;
+                    int r, c;
+                    tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, c);
                     float v = acc[e];
                     if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
                     O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + c] = v;
@@ -1303,7 +1331,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
             int p = threadIdx.x + e * nt;
             val[e] = 0.f;
             if (p < npts) {
-                int r = p / t0.nc, c = p - r * t0.nc;
+                int r, c;
+                tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, c);
                 float v = V[(size_t)(t0.r0 + r) * g.v_row + t0.c0 + c];
                 if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
                 val[e] = v;
@@ -1321,7 +1350,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
             for (int e = 0; e < PPT; ++e) {
                 int p = threadIdx.x + e * nt;
                 if (p < npts && val[e] != 0.f) {
-                    int r = p / t.nc, cidx = p - r * t.nc;
+                    int r, cidx;
+                    tile_rc(p, t.nc, 1.0f / (float)t.nc, r, cidx);
                     float2 c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
                     if (c.x == c.x) {
                         float fx = floorf(c.x), fy = floorf(c.y);

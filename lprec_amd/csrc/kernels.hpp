@@ -90,6 +90,10 @@ struct Gather {
 };
 constexpr int kGatherThreads = 256;
 constexpr int kGatherPPT = 8;   // points per thread: tiles hold <= 2048 points
+#ifndef LP_GATHER_PIPE
+#define LP_GATHER_PIPE 2
+#endif
+constexpr int kGatherPipe = LP_GATHER_PIPE;   // points whose coordinates are in flight ahead of the sampling
 // kind: 0 image->LP, 1 sinogram->LP, 2 LP->polar, 3 LP->Cartesian (names the kernel instance)
 void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hipStream_t s);
 
