@@ -203,17 +203,20 @@ bool tile_box(Coord coord, int span, int r0, int c0, int nr, int nc, int& x0, in
 }
 
 // Tiles of the output rectangle rows [rb, re) x cols [0, nc_all), split until
-// every span's box fits `cap` floats.  `spans` tiles are emitted per output
+// every span's box fits `cap` floats (column counts stay powers of two: the
+// rows are halved, or the columns).  `spans` tiles are emitted per output
 // tile (one per span, consecutive).  keep(r0, c0, nr, nc) may drop a tile.
 template <class Coord, class Keep>
 void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>& spans, int r0, int c0, int nr,
                int nc, int cap, Keep keep, int& maxbox) {
     if (!keep(r0, c0, nr, nc)) return;
+    if (nc <= 0 || (nc & (nc - 1)) || nc > lpk::kGatherThreads || (long long)nr * nc > lpk::kGatherThreads * lpk::kGatherPPT)
+        throw std::runtime_error("gather tile shape out of range");
     std::vector<lpk::GTile> t(spans.size());
     bool fits = true;
     for (size_t i = 0; i < spans.size(); ++i) {
         lpk::GTile& g = t[i];
-        g.r0 = r0; g.c0 = c0; g.nr = nr; g.nc = nc; g.span = spans[i]; g.pad = 0;
+        g.r0 = r0; g.c0 = c0; g.nr = nr; g.nc = nc; g.span = spans[i]; g.lnc = __builtin_ctz((unsigned)nc);
         tile_box(coord, spans[i], r0, c0, nr, nc, g.x0, g.y0, g.bw, g.bh);
         if ((long long)(g.bw | 1) * (g.bh | 1) > cap) fits = false;   // either box layout fits
     }
@@ -241,8 +244,13 @@ std::vector<lpk::GTile> tile_grid(Coord coord, const std::vector<int>& spans, in
                                   int cap, Keep keep, int& maxbox) {
     std::vector<lpk::GTile> out;
     for (int r = rb; r < re; r += TR)
-        for (int c = 0; c < ncols; c += TC)
-            add_tiles(out, coord, spans, r, c, std::min(TR, re - r), std::min(TC, ncols - c), cap, keep, maxbox);
+        for (int c = 0, w; c < ncols; c += w) {
+            // TC is a power of two; a ragged last column block is cut into
+            // power-of-two widths
+            w = std::min(TC, ncols - c);
+            while (w & (w - 1)) w &= w - 1;
+            add_tiles(out, coord, spans, r, c, std::min(TR, re - r), w, cap, keep, maxbox);
+        }
     return out;
 }
 

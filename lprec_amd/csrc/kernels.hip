@@ -325,9 +325,17 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // KIND only names the instance (0 image->LP, 1 sinogram->LP, 2 LP->polar,
 // 3 LP->Cartesian) so that profiles separate the four gathers.
 
-// Stage the box [x0, x0+bw) x [y0, y0+bh) of the periodic grid S [H][W] into
-// LDS rows of stride bw|1, 8 loads in flight per lane, 32-bit offsets from
-// the block-uniform base.
+// Loads and stores at a block-uniform base plus a zero-extended 32-bit byte
+// offset per lane (the base stays in scalar registers, no 64-bit lane math).
+template <class T>
+__device__ __forceinline__ T ld_at(const T* base, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T* base, unsigned off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
 // Row and column of point p of a tile with nc columns, from a float
 // reciprocal (exact for p, nc < 2^20) instead of an integer division.
 __device__ __forceinline__ void tile_rc(int p, int nc, float inv_nc, int& r, int& c) {
@@ -335,16 +343,40 @@ __device__ __forceinline__ void tile_rc(int p, int nc, float inv_nc, int& r, int
     c = p - r * nc;
 }
 
-// COLMAJOR stores the box transposed (column stride bh|1): the gathers whose
-// neighbouring samples step along the source rows (sinogram -> LP, LP ->
-// polar) then read consecutive LDS banks.
-template <int UL, bool COLMAJOR>
+// Stage the box [x0, x0+bw) x [y0, y0+bh) of the periodic grid S [H][W] into
+// LDS rows of stride bw|1, UL loads in flight per lane.  A box inside the grid
+// (nearly all of them) needs no wrapping: element i = yy bw + xx sits at
+// source offset i + yy (W - bw) and LDS offset i + yy ((bw|1) - bw).
+template <int UL>
 __device__ __forceinline__ void load_box(float* box, const float* __restrict__ S, const GTile& t, int W, int H) {
-    const int nt = blockDim.x;
+    constexpr int nt = kGatherThreads;
     const int nbox = t.bw * t.bh;
-    const int bs = t.bw | 1, cs = t.bh | 1;
+    const int bs = t.bw | 1;
     // (i + 0.5) / bw in float is exact enough for i, bw < 2^20
     const float inv = 1.0f / (float)t.bw;
+    if (t.x0 >= 0 && t.y0 >= 0 && t.x0 + t.bw <= W && t.y0 + t.bh <= H) {
+        const float* Sb = S + (size_t)t.y0 * W + t.x0;
+        const int sw = W - t.bw, lw = bs - t.bw;
+        for (int i0 = threadIdx.x; i0 < nbox; i0 += UL * nt) {
+            float v[UL];
+            int li[UL];
+#pragma unroll
+            for (int u = 0; u < UL; ++u) {
+                const int i = i0 + u * nt;
+                v[u] = 0.f;
+                li[u] = 0;
+                if (i < nbox) {
+                    const int yy = (int)(((float)i + 0.5f) * inv);
+                    v[u] = ld_at(Sb, (unsigned)(i + yy * sw) * 4u);
+                    li[u] = i + yy * lw;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UL; ++u)
+                if (i0 + u * nt < nbox) box[li[u]] = v[u];
+        }
+        return;
+    }
     for (int i0 = threadIdx.x; i0 < nbox; i0 += UL * nt) {
         float v[UL];
 #pragma unroll
@@ -363,23 +395,20 @@ __device__ __forceinline__ void load_box(float* box, const float* __restrict__ S
             int i = i0 + u * nt;
             if (i < nbox) {
                 int yy = (int)(((float)i + 0.5f) * inv);
-                int xx = i - yy * t.bw;
-                box[COLMAJOR ? xx * cs + yy : yy * bs + xx] = v[u];
+                box[i + yy * (bs - t.bw)] = v[u];
             }
         }
     }
 }
 
-// One sample of the staged box at source coordinate c (valid).
-template <bool COLMAJOR>
+// One sample of the staged box (row stride bw|1) at source coordinate c (valid).
 __device__ __forceinline__ float box_sample(const float* box, const GTile& t, float2 c, int cubic) {
     float fx = floorf(c.x), fy = floorf(c.y);
     float ax = c.x - fx, ay = c.y - fy;
-    // element (x, y) of the box sits at x * sx + y * sy
-    const int sx = COLMAJOR ? (t.bh | 1) : 1, sy = COLMAJOR ? 1 : (t.bw | 1);
+    const int sy = t.bw | 1;
     int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
     int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
-    const float* q = box + ly * sy + lx * sx;
+    const float* q = box + ly * sy + lx;
     if (cubic) {
         float wx[4], wy[4];
         bspline_w(ax, wx);
@@ -388,20 +417,23 @@ __device__ __forceinline__ float box_sample(const float* box, const GTile& t, fl
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const float* qr = q + b * sy;
-            v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[sx] + wx[2] * qr[2 * sx] + wx[3] * qr[3 * sx]);
+            v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
         }
         return v;
     }
-    const float* q1 = q + sy + sx;
+    const float* q1 = q + sy + 1;
     const float* q2 = q1 + sy;
-    return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[sx]) + ay * ((1.f - ax) * q2[0] + ax * q2[sx]);
+    return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
 }
 
 template <int KIND, int GROUP>
 __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile* __restrict__ tiles) {
     extern __shared__ float box[];
     constexpr int PPT = kGatherPPT;
-    const int nt = blockDim.x;
+    // coordinates (and scales) run GP points ahead of the sampling; the first
+    // GP are issued before the box load and arrive while it streams in
+    constexpr int GP = kGatherPipe;
+    static_assert(PPT % GP == 0, "points per thread come in groups of GP");
     // logical order: the slice groups of one tile are neighbours (they share
     // the coordinates), then neighbouring tiles (they share source rows)
     const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
@@ -409,48 +441,40 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
     const int zg = lb % nzg, tile = lb / nzg;
     const GTile* tb = tiles + (size_t)tile * GROUP;
     const GTile t0 = tb[0];
-    const int npts = t0.nr * t0.nc;
+    // the tile has nc = 2^lnc columns: lane (lr, lc) holds the points of rows
+    // lr + e * rstep, column lc, so every per-point address is a uniform base
+    // (row e * rstep) plus a fixed lane offset
+    const int lr = threadIdx.x >> t0.lnc, lc = threadIdx.x & (t0.nc - 1);
+    const int rstep = kGatherThreads >> t0.lnc;
+    const unsigned o_lane = (unsigned)(lr * g.o_row + lc) * 4u;
+    const unsigned c_lane = (unsigned)(lr * g.c_row + lc) * 8u;
+    const unsigned s_lane = (unsigned)(lr * g.s_row + lc) * 4u;
     const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    const float* sb = g.scale ? g.scale + (size_t)t0.r0 * g.s_row + t0.c0 : nullptr;
     for (int z = z0; z < z1; ++z) {
-        float* O = g.out + (size_t)z * g.o_z + (size_t)t0.span * g.o_span;
+        float* O = g.out + (size_t)z * g.o_z + (size_t)t0.span * g.o_span + (size_t)t0.r0 * g.o_row + t0.c0;
         float acc[GROUP > 1 ? PPT : 1];
 #pragma unroll
         for (int e = 0; e < (GROUP > 1 ? PPT : 1); ++e) acc[e] = 0.f;
 #pragma unroll 1
         for (int gi = 0; gi < GROUP; ++gi) {
             const GTile t = tb[gi];
-            constexpr bool CM = false;   // row-major boxes (column-major measured slower for the sinogram gather)
-            // coordinates (and e^rho scales) run GP points ahead of the sampling;
-            // the first GP are issued before the box load and arrive while it
-            // streams in
-            constexpr int GP = kGatherPipe;
-            static_assert(PPT % GP == 0, "points per thread come in groups of GP");
-            const float2* cbase = g.coord + (size_t)t.span * g.c_span;
+            const float2* cb = g.coord + (size_t)t.span * g.c_span + (size_t)(t.r0 + g.c_ro) * g.c_row + t.c0 + g.c_co;
             auto coord_of = [&](int e) {
-                int p = threadIdx.x + e * nt;
                 float2 c = make_float2(__int_as_float(0x7fc00000), 0.f);   // NaN: no sample
-                if (p < npts) {
-                    int r, cidx;
-                    tile_rc(p, t.nc, 1.0f / (float)t.nc, r, cidx);
-                    c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
-                }
+                if (lr + e * rstep < t0.nr) c = ld_at(cb + (size_t)(e * rstep) * g.c_row, c_lane);
                 return c;
             };
             auto scale_of = [&](int e) {
-                int p = threadIdx.x + e * nt;
                 float sc = 1.f;
-                if (GROUP == 1 && g.scale && p < npts) {
-                    int r, cidx;
-                    tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, cidx);
-                    sc = g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
-                }
+                if (GROUP == 1 && sb && lr + e * rstep < t0.nr) sc = ld_at(sb + (size_t)(e * rstep) * g.s_row, s_lane);
                 return sc;
             };
             float2 cur[GP];
             float scur[GP];
 #pragma unroll
             for (int u = 0; u < GP; ++u) { cur[u] = coord_of(u); scur[u] = scale_of(u); }
-            if (t.bw > 0) load_box<8, CM>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
+            if (t.bw > 0) load_box<8>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
             __syncthreads();
 #pragma unroll 1
             for (int e = 0; e < PPT; e += GP) {
@@ -465,15 +489,11 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
                 float vals[GP];
 #pragma unroll
                 for (int u = 0; u < GP; ++u) {
-                    int p = threadIdx.x + (e + u) * nt;
+                    const bool in = lr + (e + u) * rstep < t0.nr;
                     const float2 c = cur[u];
-                    float v = (p < npts && c.x == c.x) ? box_sample<CM>(box, t, c, g.cubic) : 0.f;
+                    float v = (in && c.x == c.x) ? box_sample(box, t, c, g.cubic) : 0.f;
                     vals[u] = v;
-                    if (GROUP == 1 && p < npts) {
-                        int r, cidx;
-                        tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, cidx);
-                        O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + cidx] = v * scur[u];
-                    }
+                    if (GROUP == 1 && in) st_at(O + (size_t)((e + u) * rstep) * g.o_row, o_lane, v * scur[u]);
                 }
                 if constexpr (GROUP > 1) {
                     // acc is indexed by a loop variable: keep it in registers by
@@ -490,16 +510,13 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
             }
             __syncthreads();
         }
-        if (GROUP > 1) {
+        if constexpr (GROUP > 1) {
 #pragma unroll
             for (int e = 0; e < PPT; ++e) {
-                int p = threadIdx.x + e * nt;
-                if (p < npts) {
-                    int r, c;
-                    tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, c);
+                if (lr + e * rstep < t0.nr) {
                     float v = acc[e];
-                    if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
-                    O[(size_t)(t0.r0 + r) * g.o_row + t0.c0 + c] = v;
+                    if (sb) v *= ld_at(sb + (size_t)(e * rstep) * g.s_row, s_lane);
+                    st_at(O + (size_t)(e * rstep) * g.o_row, o_lane, v);
                 }
             }
         }

@@ -65,11 +65,13 @@ void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, con
 // ---- LDS-tiled gathers ---------------------------------------------------------
 // One output tile (nr x nc points at (r0, c0)) and the source box [x0, x0+bw) x
 // [y0, y0+bh) (unwrapped, periodic in the source) that holds every tap of its
-// points for span `span`.  Built on the host from the coordinate tables.
+// points for span `span`.  Built on the host from the coordinate tables; nc is
+// a power of two (nc = 2^lnc <= kGatherThreads) and nr * nc <= kGatherThreads *
+// kGatherPPT.
 struct GTile {
     int r0, c0, nr, nc;
     int x0, y0, bw, bh;
-    int span, pad;
+    int span, lnc;
 };
 // out[z*o_z + span*o_span + r*o_row + c] = (sum over the `group` tiles of a
 // block of) sample(src + z*src_z + span*src_span, coord[span*c_span +
