@@ -748,45 +748,72 @@ __device__ __forceinline__ void row_sync() {
     else __syncthreads();
 }
 
-// r2c_post / c2r_pre of one row by the T16 lanes that own it
-__device__ __forceinline__ void r2c_post_row(float2* row, int h, const float2* __restrict__ twN, int li, int nl) {
-    for (int k = li; k <= h / 2; k += nl) {
+// r2c_post / c2r_pre of one row by the T16 lanes that own it, unrolled:
+// lane li owns the pairs (k, H - k) for k = li + i T16 < H/2 and lane 0 also
+// k = H/2.  Every slot is read and written by its owner only, so no barrier
+// separates the reads from the writes; the twiddle loads are all issued
+// before the first use.
+template <int H, int T16>
+__device__ __forceinline__ void r2c_post_row(float2* row, const float2* __restrict__ twN, int li) {
+    constexpr int NI = H / (2 * T16);
+    float2 W[NI], A[NI], B[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int k = li + i * T16;
+        W[i] = twN[k];
+        A[i] = row[lidx(k)];
+        B[i] = row[lidx(k == 0 ? 0 : H - k)];
+    }
+    float2 Am = make_float2(0.f, 0.f), Wm = make_float2(0.f, 0.f);
+    if (li == 0) { Am = row[lidx(H / 2)]; Wm = twN[H / 2]; }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int k = li + i * T16;
         if (k == 0) {
-            float2 z = row[0];
-            row[0] = make_float2(z.x + z.y, z.x - z.y);
+            row[0] = make_float2(A[i].x + A[i].y, A[i].x - A[i].y);
         } else {
-            int kk = h - k;
-            float2 A = row[lidx(k)], B = row[lidx(kk)];
-            float2 Bc = cconj(B);
-            float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
-            float2 D = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
-            float2 O = make_float2(D.y, -D.x);
-            float2 WO = cmul(twN[k], O);
+            float2 Bc = cconj(B[i]);
+            float2 E = make_float2(0.5f * (A[i].x + Bc.x), 0.5f * (A[i].y + Bc.y));
+            float2 D = make_float2(0.5f * (A[i].x - Bc.x), 0.5f * (A[i].y - Bc.y));
+            float2 WO = cmul(W[i], make_float2(D.y, -D.x));
             row[lidx(k)] = cadd(E, WO);
-            if (kk != k) row[lidx(kk)] = cconj(csub(E, WO));
+            row[lidx(H - k)] = cconj(csub(E, WO));
         }
     }
+    if (li == 0) {
+        // k = H/2 pairs with itself: E = Re A, D = i Im A
+        float2 WO = cmul(Wm, make_float2(Am.y, 0.f));
+        row[lidx(H / 2)] = cadd(make_float2(Am.x, 0.f), WO);
+    }
 }
-__device__ __forceinline__ void c2r_pre_row(float2* row, int h, const float2* __restrict__ twN, int li, int nl) {
-    for (int k = li; k <= h / 2; k += nl) {
+template <int H, int T16>
+__device__ __forceinline__ void c2r_pre_row(float2* row, const float2* __restrict__ twN, int li) {
+    constexpr int NI = H / (2 * T16);
+    float2 W[NI], P[NI], Q[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int k = li + i * T16;
+        W[i] = twN[k];
+        P[i] = row[lidx(k)];
+        Q[i] = row[lidx(k == 0 ? 0 : H - k)];
+    }
+    float2 Pm = make_float2(0.f, 0.f), Wm = make_float2(0.f, 0.f);
+    if (li == 0) { Pm = row[lidx(H / 2)]; Wm = twN[H / 2]; }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int k = li + i * T16;
         if (k == 0) {
-            float2 p = row[0];
-            row[0] = make_float2(p.x + p.y, p.x - p.y);
+            row[0] = make_float2(P[i].x + P[i].y, P[i].x - P[i].y);
         } else {
-            int kk = h - k;
-            float2 P = row[lidx(k)], Q = row[lidx(kk)];
-            float2 S = cadd(P, cconj(Q));
-            float2 D = csub(P, cconj(Q));
-            float2 w = twN[k];
-            float2 iwc = make_float2(w.y, w.x);
-            row[lidx(k)] = cadd(S, cmul(iwc, D));
-            if (kk != k) {
-                float2 S2 = cconj(S);
-                float2 D2 = cconj(make_float2(-D.x, -D.y));
-                float2 imw = make_float2(w.y, -w.x);
-                row[lidx(kk)] = cadd(S2, cmul(imw, D2));
-            }
+            float2 S = cadd(P[i], cconj(Q[i]));
+            float2 D = csub(P[i], cconj(Q[i]));
+            row[lidx(k)] = cadd(S, cmul(make_float2(W[i].y, W[i].x), D));
+            row[lidx(H - k)] = cadd(cconj(S), cmul(make_float2(W[i].y, -W[i].x), cconj(make_float2(-D.x, -D.y))));
         }
+    }
+    if (li == 0) {
+        float2 S = cadd(Pm, cconj(Pm)), D = csub(Pm, cconj(Pm));
+        row[lidx(H / 2)] = cadd(S, cmul(make_float2(Wm.y, Wm.x), D));
     }
 }
 
@@ -840,6 +867,7 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
     // accesses rr * RS + lidx(f) of a half-wave cover R rows x 32/R columns,
     // which then land on 32 distinct 8-byte bank pairs
     const int RS = H + H / 16 + 32 / R;
+    const int lgR = __builtin_ctz((unsigned)R);   // R is a power of two
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
     // logical order: neighbouring row blocks (which share the 128-byte lines
     // of the transposed spectra) run behind the same L2
@@ -877,12 +905,21 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
         row_sync<WAVE>();
-        r2c_post_row(rowbuf, H, rp.twN, li, T16);
+        r2c_post_row<H, T16>(rowbuf, rp.twN, li);
         __syncthreads();   // the transposed store reads every row
-        float2* out = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
-        for (int qq = threadIdx.x; qq < R * H; qq += blockDim.x) {
-            int f = qq / R, rr = qq - f * R;
-            out[(size_t)f * Nr + rr] = smc[rr * RS + lidx(f)];
+        {
+            // thread (rr, f0) = (tid mod R, tid / R) stores frequencies f0 + i T16
+            // (i < 16) of row rr: LDS slots at fixed steps, global addresses a
+            // uniform base per i plus one lane offset
+            const int rr = threadIdx.x & (R - 1), f0 = threadIdx.x >> lgR;
+            const float2* src = smc + rr * RS + lidx(f0);
+            float2* out = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
+            const unsigned lane = (unsigned)(f0 * Nr + rr) * 8u;
+            float2 w[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) w[i] = src[i * (T16 + T16 / 16)];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) st_at(out + (size_t)(i * T16) * Nr, lane, w[i]);
         }
         __syncthreads();
     }
@@ -896,6 +933,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
     constexpr bool WAVE = T16 <= 64;
     const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
     const int RS = H + H / 16 + 32 / R;
+    const int lgR = __builtin_ctz((unsigned)R);   // R is a power of two
     const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
     const int nrb = Nr / R, nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -908,12 +946,20 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
     const int e0 = max(wr.x, 0) >> 1, e1 = min((wr.y + 1) >> 1, H);
     float2* rowbuf = smc + r * RS;
     for (int z = z0; z < z1; ++z) {
-        const float2* in = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
-        stage_in(
-            R * H, [&](int qq) { int f = qq / R, rr = qq - f * R; return in[(size_t)f * Nr + rr]; },
-            [&](int qq, float2 v) { int f = qq / R, rr = qq - f * R; smc[rr * RS + lidx(f)] = v; });
+        {
+            // transposed load, the mirror of k_rows16_fwd's store
+            const int rr = threadIdx.x & (R - 1), f0 = threadIdx.x >> lgR;
+            float2* dst = smc + rr * RS + lidx(f0);
+            const float2* in = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
+            const unsigned lane = (unsigned)(f0 * Nr + rr) * 8u;
+            float2 w[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) w[i] = ld_at(in + (size_t)(i * T16) * Nr, lane);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) dst[i * (T16 + T16 / 16)] = w[i];
+        }
         __syncthreads();   // every row was written by the whole block
-        c2r_pre_row(rowbuf, H, rp.twN, li, T16);
+        c2r_pre_row<H, T16>(rowbuf, rp.twN, li);
         row_sync<WAVE>();
         float2 v[16];
 #pragma unroll
