@@ -319,6 +319,96 @@ __device__ bool fft_stage_oddsym(float2* buf, int ldm, int nlines, int M, int Ns
     return true;
 }
 
+// The same stage for R <= 47 with the weights tabulated per term:
+// wq[(q - 1) GP + p - 1] = w_{(p q) mod R} (GP = G PP columns, zero past hR), so
+// the PP weights a work item needs for term q are consecutive and come in
+// 16-byte pairs, with no modular index arithmetic in the term loop.  One work
+// item per thread; returns false when that does not fit.
+template <int EPT>
+__device__ bool fft_stage_oddsym_t(float2* buf, int ldm, int nlines, int M, int Ns, int R,
+                                   const float2* __restrict__ tw, int dir) {
+    constexpr int PP = 8, MAXR = 47, MAXH = (MAXR - 1) / 2, MAXGP = ((MAXH + PP - 1) / PP) * PP;
+    const int nbf = M / R, hR = (R - 1) / 2;
+    const int G = (hR + PP - 1) / PP, GP = G * PP;
+    const int total = nlines * nbf * G;
+    const int nt = blockDim.x;
+    if (R > MAXR || (R & 1) == 0 || total > nt) return false;
+    __shared__ __align__(16) float2 wq[MAXH * MAXGP];
+    for (int e = threadIdx.x; e < hR * GP; e += nt) {
+        const int q = e / GP + 1, p = e - (q - 1) * GP + 1;
+        wq[e] = p <= hR ? twiddle(tw, ((p * q) % R) * nbf, dir) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int tstep = M / (Ns * R);
+    const int t = threadIdx.x;
+    float2 ya[PP], yb[PP], y0 = make_float2(0.f, 0.f);
+    int g = 0, dst = 0;
+    if (t < total) {
+        g = t % G;
+        const int bi = t / G;
+        const int b = bi / nbf, i = bi - b * nbf;
+        const int k = i % Ns;
+        const int src = b * ldm + i;
+        dst = b * ldm + (i - k) * R + k;
+        const float2 x0 = buf[lidx(src)];
+        float2 A[PP], B[PP];
+#pragma unroll
+        for (int j = 0; j < PP; ++j) { A[j] = x0; B[j] = make_float2(0.f, 0.f); }
+        float2 S0 = x0;
+        const int e1 = k * tstep;
+        const float2 w1 = twiddle(tw, e1, dir);
+        const float2 w1c = make_float2(w1.x, -w1.y);
+        float2 wa = make_float2(1.f, 0.f), wb = make_float2(1.f, 0.f);
+        const float4* wrow = reinterpret_cast<const float4*>(wq + g * PP);
+        for (int q = 1; q <= hR; ++q) {
+            float2 xa = buf[lidx(src + q * nbf)], xb = buf[lidx(src + (R - q) * nbf)];
+            if (Ns > 1) {
+                if (((q - 1) & 7) == 0) {
+                    wa = twiddle(tw, (int)(((long long)q * e1) % M), dir);
+                    wb = twiddle(tw, (int)(((long long)(R - q) * e1) % M), dir);
+                }
+                xa = cmul(xa, wa);
+                xb = cmul(xb, wb);
+                wa = cmul(wa, w1);
+                wb = cmul(wb, w1c);
+            }
+            const float2 sq = cadd(xa, xb), dq = csub(xa, xb);
+            S0 = cadd(S0, sq);
+            const float4* wr = wrow + (q - 1) * (GP / 2);
+#pragma unroll
+            for (int jj = 0; jj < PP / 2; ++jj) {
+                const float4 w2 = wr[jj];
+                A[2 * jj].x += sq.x * w2.x; A[2 * jj].y += sq.y * w2.x;
+                B[2 * jj].x += dq.x * w2.y; B[2 * jj].y += dq.y * w2.y;
+                A[2 * jj + 1].x += sq.x * w2.z; A[2 * jj + 1].y += sq.y * w2.z;
+                B[2 * jj + 1].x += dq.x * w2.w; B[2 * jj + 1].y += dq.y * w2.w;
+            }
+        }
+        y0 = S0;
+#pragma unroll
+        for (int j = 0; j < PP; ++j) {
+            const float2 iB = make_float2(-B[j].y, B[j].x);
+            ya[j] = cadd(A[j], iB);
+            yb[j] = csub(A[j], iB);
+        }
+    }
+    __syncthreads();
+    if (t < total) {
+        const int p0 = 1 + g * PP;
+        if (g == 0) buf[lidx(dst)] = y0;
+#pragma unroll
+        for (int j = 0; j < PP; ++j) {
+            const int p = p0 + j;
+            if (p <= hR) {
+                buf[lidx(dst + p * Ns)] = ya[j];
+                buf[lidx(dst + (R - p) * Ns)] = yb[j];
+            }
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
 // Full in-place FFT (dir = -1 forward, +1 inverse, unnormalised) of nlines
 // lines of length plan.M.  Ends with a barrier.
 template <int EPT>
@@ -334,7 +424,8 @@ __device__ void fft_lds(float2* buf, int ldm, int nlines, const FftPlan& plan, c
             case 5: fft_stage<5, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
             case 8: fft_stage<8, EPT>(buf, ldm, nlines, M, Ns, tw, dir); break;
             default:
-                if (!fft_stage_oddsym<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir))
+                if (!fft_stage_oddsym_t<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir) &&
+                    !fft_stage_oddsym<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir))
                     fft_stage_generic<EPT>(buf, ldm, nlines, M, Ns, R, tw, dir);
                 break;
         }

@@ -42,8 +42,10 @@ FftPlan make_plan(int M) {
     while (n % 4 == 0) { r.push_back(4); n /= 4; }
     while (n % 2 == 0) { r.push_back(2); n /= 2; }
     for (int q : {3, 5}) while (n % q == 0) { r.push_back(q); n /= q; }
-    for (int q = 7; (long long)q * q <= n; q += 2) while (n % q == 0) { r.push_back(q); n /= q; }
-    if (n > 1) r.push_back(n);
+    std::vector<int> big;   // factors > 5 run first: their stage then needs no twiddles
+    for (int q = 7; (long long)q * q <= n; q += 2) while (n % q == 0) { big.push_back(q); n /= q; }
+    if (n > 1) big.push_back(n);
+    r.insert(r.begin(), big.rbegin(), big.rend());
     if ((int)r.size() > LP_MAX_STAGES) throw std::runtime_error("FFT length has too many factors");
     for (size_t i = 0; i < r.size(); ++i) p.radix[i] = r[i];
     p.nst = (int)r.size();

@@ -401,27 +401,42 @@ __device__ __forceinline__ void load_box(float* box, const float* __restrict__ S
     }
 }
 
+// Cubic B-spline weights times 6: (1-f)^3, 3f^3 - 6f^2 + 4, -3f^3 + 3f^2 +
+// 3f + 1, f^3 (the 1/36 of the two axes is applied once per sample).
+__device__ __forceinline__ void bspline_w6(float f, float* w) {
+    const float one = 1.0f - f, sq = f * f;
+    w[0] = one * one * one;
+    w[1] = fmaf(sq, fmaf(3.0f, f, -6.0f), 4.0f);
+    w[2] = fmaf(f, fmaf(f, fmaf(-3.0f, f, 3.0f), 3.0f), 1.0f);
+    w[3] = sq * f;
+}
+
 // One sample of the staged box (row stride bw|1) at source coordinate c (valid).
+// The LDS slot of tap (0, 0) is formed in float from box-relative integers
+// (exact below 2^24) and clamped into the box in one step; the tile boxes hold
+// every tap of their points, so the clamp only guards the LDS bounds.
 __device__ __forceinline__ float box_sample(const float* box, const GTile& t, float2 c, int cubic) {
-    float fx = floorf(c.x), fy = floorf(c.y);
-    float ax = c.x - fx, ay = c.y - fy;
+    const float fx = floorf(c.x), fy = floorf(c.y);
+    const float ax = c.x - fx, ay = c.y - fy;
     const int sy = t.bw | 1;
-    int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
-    int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
-    const float* q = box + ly * sy + lx;
     if (cubic) {
+        float slot = fmaf(fy - (float)(t.y0 + 1), (float)sy, fx - (float)(t.x0 + 1));
+        slot = __builtin_amdgcn_fmed3f(slot, 0.f, (float)((t.bh - 4) * sy + t.bw - 4));
+        const float* q = box + (int)slot;
         float wx[4], wy[4];
-        bspline_w(ax, wx);
-        bspline_w(ay, wy);
+        bspline_w6(ax, wx);
+        bspline_w6(ay, wy);
         float v = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const float* qr = q + b * sy;
             v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
         }
-        return v;
+        return v * (1.0f / 36.0f);
     }
-    const float* q1 = q + sy + 1;
+    float slot = fmaf(fy - (float)t.y0, (float)sy, fx - (float)t.x0);
+    slot = __builtin_amdgcn_fmed3f(slot, 0.f, (float)((t.bh - 2) * sy + t.bw - 2));
+    const float* q1 = box + (int)slot;
     const float* q2 = q1 + sy;
     return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
 }
@@ -730,6 +745,138 @@ __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __r
             for (int q = 0; q < 16; ++q) col[li + q * T16] = v[q];
         }
         __syncthreads();
+    }
+}
+
+// Block barrier that waits only for this wave's LDS traffic (lgkmcnt), not
+// for its outstanding global loads: the load-to-LDS stream of the next column
+// keeps running across it.  Compiler-level ordering from the wave-scope fences.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// w^q (q = 1..15) applied to v[q] from w, w^2, w^4, w^8 held in registers
+// (conjugated for the inverse)
+__device__ __forceinline__ void twiddle16_w(float2* v, const float2* w, int dir) {
+    const bool cj = dir > 0;
+    float2 w1 = cj ? cconj(w[0]) : w[0], w2 = cj ? cconj(w[1]) : w[1];
+    float2 w4 = cj ? cconj(w[2]) : w[2], w8 = cj ? cconj(w[3]) : w[3];
+    float2 w3 = cmul(w1, w2);
+    v[1] = cmul(v[1], w1);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], cmul(w1, w4));
+    v[6] = cmul(v[6], cmul(w2, w4));
+    v[7] = cmul(v[7], cmul(w3, w4));
+    v[8] = cmul(v[8], w8);
+    v[9] = cmul(v[9], cmul(w1, w8));
+    v[10] = cmul(v[10], cmul(w2, w8));
+    v[11] = cmul(v[11], cmul(w3, w8));
+    float2 w12 = cmul(w4, w8);
+    v[12] = cmul(v[12], w12);
+    v[13] = cmul(v[13], cmul(w1, w12));
+    v[14] = cmul(v[14], cmul(w2, w12));
+    v[15] = cmul(v[15], cmul(w3, w12));
+}
+
+// Column pass for M = 4096 (16 x 16 x 16) with the next slice's column
+// streamed from HBM into LDS (load-to-LDS, no registers) while the current one
+// is transformed.  The twiddles and the kernel-spectrum column are loaded once
+// per block, so no other global load shares the vector-memory counter with the
+// stream and every barrier inside a slice waits for LDS traffic only.
+__global__ void __launch_bounds__(256, 2) k_cols4096(RowPass rp, const float2* __restrict__ twM,
+                                                     const float2* __restrict__ fzT) {
+    constexpr int M = 4096, T16 = 256;
+    __shared__ float2 buf[lds_slots(M)];
+    __shared__ __align__(16) float2 pre[M];
+    const int h = rp.h;
+    const int li = threadIdx.x;
+    const int nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int zg = lb % nzg, t = lb / nzg;
+    const int span = t % 3, f = 1 + t / 3;
+    const int z0 = zg * rp.zgroup;
+    const int z1 = min(z0 + rp.zgroup, rp.nz);
+    const int wv = li >> 6, ln = li & 63;
+    auto prefetch = [&](int z) {
+        const float2* col = rp.S + (((size_t)span * rp.nz + z) * h + f) * (size_t)M;
+#pragma unroll
+        for (int c = 0; c < M / 128 / 4; ++c) {
+            const int cc = wv + 4 * c;
+            __builtin_amdgcn_global_load_lds(col + cc * 128 + ln * 2, pre + cc * 128, 16, 0, 0);
+        }
+    };
+    prefetch(z0);
+    // per-block constants: middle-stage twiddles (k = li & 15, step 16),
+    // last-stage twiddles (k = li, step 1), the kernel-spectrum column
+    float2 wm[4], wl[4], kz[16];
+    {
+        const int km = (li & 15) * 16;
+        wm[0] = twM[km]; wm[1] = twM[2 * km]; wm[2] = twM[4 * km]; wm[3] = twM[8 * km];
+        wl[0] = twM[li]; wl[1] = twM[2 * li]; wl[2] = twM[4 * li]; wl[3] = twM[8 * li];
+        const float2* fz = fzT + (size_t)f * M;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) kz[q] = fzval(fz, li + q * T16, rp.conj);
+    }
+    for (int z = z0; z < z1; ++z) {
+        float2* col = rp.S + (((size_t)span * rp.nz + z) * h + f) * (size_t)M;
+        float2 v[16];
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();   // column z is in pre
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = pre[li + q * T16];
+        dft16(v, -1);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) buf[lidx(li * 16 + q)] = v[q];
+        lds_barrier();   // pre has been read by every wave
+        if (z + 1 < z1) prefetch(z + 1);
+        // middle stage (radix 16, Ns = 16)
+        {
+            const int j = li, k = j & 15;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = buf[lidx(j + q * 256)];
+            lds_barrier();
+            twiddle16_w(v, wm, -1);
+            dft16(v, -1);
+            const int dst = (j - k) * 16 + k;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) buf[lidx(dst + q * 16)] = v[q];
+            lds_barrier();
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = buf[lidx(li + q * T16)];
+        twiddle16_w(v, wl, -1);
+        dft16(v, -1);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], kz[q]);
+        dft16(v, +1);
+        lds_barrier();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) buf[lidx(li * 16 + q)] = v[q];
+        lds_barrier();
+        {
+            const int j = li, k = j & 15;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = buf[lidx(j + q * 256)];
+            lds_barrier();
+            twiddle16_w(v, wm, +1);
+            dft16(v, +1);
+            const int dst = (j - k) * 16 + k;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) buf[lidx(dst + q * 16)] = v[q];
+            lds_barrier();
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = buf[lidx(li + q * T16)];
+        twiddle16_w(v, wl, +1);
+        dft16(v, +1);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) col[li + q * T16] = v[q];
+        lds_barrier();
     }
 }
 
@@ -1170,7 +1317,7 @@ void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, con
                 case 2: hipLaunchKernelGGL(k_cols16<2>, grid, block, lds, s, rp, twM, fzT); break;
                 case 4: hipLaunchKernelGGL(k_cols16<4>, grid, block, lds, s, rp, twM, fzT); break;
                 case 8: hipLaunchKernelGGL(k_cols16<8>, grid, block, lds, s, rp, twM, fzT); break;
-                default: hipLaunchKernelGGL(k_cols16<16>, grid, block, lds, s, rp, twM, fzT); break;
+                default: hipLaunchKernelGGL(k_cols4096, grid, block, 0, s, rp, twM, fzT); break;   // static LDS
             }
         }
         return;
