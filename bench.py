@@ -125,20 +125,20 @@ def pmc_summary(root, copy_mib):
 # recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
 MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536916858, 536870912),
-    "fwd.col_iir": (795739678, 536870912),
-    "fwd.gather_lp": (4494355256, 1133599808),
-    "fwd.rows_fwd": (1097154233, 3350206048),
-    "fwd.cols": (3256788322, 3221422080),
-    "fwd.rows_inv": (3230251711, 1623203008),
-    "fwd.polar_out": (4653016336, 393216000),
-    "adj.sino_rows": (393255296, 786432000),
-    "adj.col_iir": (1168477435, 786432000),
-    "adj.gather_lp": (4088179137, 1610832848),
-    "adj.rows_fwd": (1617652836, 3413262496),
-    "adj.cols": (3256744802, 3221422080),
-    "adj.rows_inv": (3227553884, 1084601456),
-    "adj.cart_out": (3545455366, 536870912),
+    "fwd.prep_rows": (536916543, 536870912),
+    "fwd.col_iir": (1025694171, 536870912),
+    "fwd.gather_lp": (4428803155, 1133801728),
+    "fwd.rows_fwd": (1092990785, 3744383840),
+    "fwd.cols": (3258449999, 3221741632),
+    "fwd.rows_inv": (3329652687, 1624334832),
+    "fwd.polar_out": (4666527075, 393216000),
+    "adj.sino_rows": (393255564, 786432000),
+    "adj.col_iir": (1519977152, 786432000),
+    "adj.gather_lp": (4048013579, 1610861568),
+    "adj.rows_fwd": (1613190502, 3736210240),
+    "adj.cols": (3259519045, 3221741632),
+    "adj.rows_inv": (3351653621, 1085025376),
+    "adj.cart_out": (3729863959, 536870912),
 }
 
 

@@ -1301,12 +1301,15 @@ static int cols16_mid(int M) {
 void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s) {
     if (int mid = rp.fft16 ? cols16_mid(rp.Nrho) : 0) {
         const int T16 = rp.Nrho / 16;
-        // column 0 (packed theta-DC / Nyquist pair) on the generic kernel
+        // column 0 (packed theta-DC / Nyquist pair) on the generic kernel, one
+        // block per span and slice (a few blocks only: latency-bound)
         {
+            RowPass r0 = rp;
+            r0.zgroup = 1;
             int nt = threads_for(rp.Nrho);
             int ept = ept_for(rp.Nrho, nt);
-            dim3 g0(1, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
-            LP_DISPATCH_EPT(ept, k_cols, g0, dim3(nt), cols_lds_bytes(rp.Nrho), s, rp, planM, twM, fzT);
+            dim3 g0(1, 3, rp.nz);
+            LP_DISPATCH_EPT(ept, k_cols, g0, dim3(nt), cols_lds_bytes(rp.Nrho), s, r0, planM, twM, fzT);
         }
         dim3 grid((rp.h - 1) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(std::max(T16, 64));
