@@ -185,6 +185,15 @@ def _timed(fn, steps, warmup):
     return time.perf_counter() - t0
 
 
+def _schedule(args):
+    """--tune key=value,... -> schedule dict (None when empty)."""
+    sched = {}
+    for kv in filter(None, args.tune.split(",")):
+        k, v = kv.split("=")
+        sched[k] = int(v)
+    return sched or None
+
+
 def run_fbp(args):
     """FBP (adjp with a filter) of 32 noisy slices at N0 = 2048, 1500 angles and
     a centre offset drawn from U(-3, 3) px (padded grid N = 2064: FFT lengths
@@ -202,7 +211,7 @@ def run_fbp(args):
     for filt in ("ramp", "parzen"):
         lp = lpTransform(N0, P0, ns, filt, cor, "cubic")
         lp.precompute(0)
-        lp.initcmem(0, 0)
+        lp.initcmem(0, 0, schedule=_schedule(args))
         h = lp.clphandle[0]
         g = h.info()
         for _ in range(args.warmup):

@@ -90,7 +90,8 @@ int lp_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
  * int64). */
 int lp_info(const lp_handle* h, int64_t* info);
 /* Kernel schedule knobs: key 0 = log-polar rows per row-FFT block (1..16),
- * key 1 = slices per block in the row/column passes (>=1), key 2 = rows per
+ * key 1 = slices per block in the row/column passes (>=1; halved on launch
+ * while a pass would have fewer than 2048 blocks), key 2 = rows per
  * prefilter column chunk (32, 64 or 128), key 3 = slices per block in the
  * tiled gathers (>=1), key 4 = FFT kernel family (1 = radix-16 register
  * kernels where the length is 16*k*16 with k in {1,2,4,8,16}, 0 = the

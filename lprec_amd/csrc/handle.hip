@@ -59,7 +59,7 @@ struct lp_handle {
     int N = 0, N0 = 0, Nt = 0, Nr = 0, P = 0, P0 = 0, Ns = 0, cor = 0, os = 1, h = 0;
     bool cubic = true;
     bool has_fwd = false, has_adj = false, has_filter = false;
-    int R = 8, zgroup = 4, colchunk = 32, gzgroup = 4, fft16 = 1;
+    int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -363,7 +363,7 @@ int lp_create(const float* g, int device, lp_handle** out) {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         h->lds_limit = (size_t)lds;
-    h->R = 8;
+    h->R = 16;
     h->R = pick_rows(h.get());
     try {
         h->plan_h = make_plan(h->h);

@@ -1246,18 +1246,32 @@ static size_t rows16_lds(const RowPass& rp) {
     } while (0)
 
 
-void launch_rows_fwd(const RowPass& rp, const float* F, const int2* range, hipStream_t s) {
+// Slices per block, reduced for small batches so that a launch still has
+// enough blocks to fill the chip several times over.
+static RowPass fit_zgroup(const RowPass& in, long long blocks_per_slice) {
+    RowPass rp = in;
+    while (rp.zgroup > 1 && blocks_per_slice * ((rp.nz + rp.zgroup - 1) / rp.zgroup) < 2048) rp.zgroup /= 2;
+    return rp;
+}
+
+void launch_rows_fwd(const RowPass& rp0, const float* F, const int2* range, hipStream_t s) {
+    const RowPass rp = fit_zgroup(rp0, 3LL * (rp0.Nrho / rp0.R));
     if (int mid = rows16_mid(rp)) {
         dim3 grid((rp.Nrho / rp.R) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(rp.R * (rp.h / 16));
         LP_DISPATCH_MID(mid, k_rows16_fwd, grid, block, rows16_lds(rp), s, rp, F, range);
         return;
     }
-    int nt = threads_for((long long)rp.R * rp.h);
-    int ept = ept_for((long long)rp.R * rp.h, nt);
-    int ldm = rp.ldm;
-    dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
-    LP_DISPATCH_EPT(ept, k_rows_fwd, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, F, range, ldm);
+    // the mixed-radix kernels keep at most 8 rows per block (their register
+    // budget); Nrho stays divisible since R is a power of two
+    RowPass rg = rp;
+    rg.R = std::min(rg.R, 8);
+    rg = fit_zgroup(rg, 3LL * (rg.Nrho / rg.R));
+    int nt = threads_for((long long)rg.R * rg.h);
+    int ept = ept_for((long long)rg.R * rg.h, nt);
+    int ldm = rg.ldm;
+    dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
+    LP_DISPATCH_EPT(ept, k_rows_fwd, grid, dim3(nt), rows_lds_bytes(ldm, rg.R), s, rg, F, range, ldm);
 }
 
 __global__ void k_add_extras(float* __restrict__ F, int nz, int Nt, int Nr, Src src, const int* __restrict__ xrow,
@@ -1278,18 +1292,22 @@ void launch_add_extras(float* F, int nz, int Ntheta, int Nrho, const Src& src, c
                        nx, (int)cubic);
 }
 
-void launch_rows_inv(const RowPass& rp, float* G, const int2* wrange, hipStream_t s) {
+void launch_rows_inv(const RowPass& rp0, float* G, const int2* wrange, hipStream_t s) {
+    const RowPass rp = fit_zgroup(rp0, 3LL * (rp0.Nrho / rp0.R));
     if (int mid = rows16_mid(rp)) {
         dim3 grid((rp.Nrho / rp.R) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
         dim3 block(rp.R * (rp.h / 16));
         LP_DISPATCH_MID(mid, k_rows16_inv, grid, block, rows16_lds(rp), s, rp, G, wrange);
         return;
     }
-    int nt = threads_for((long long)rp.R * rp.h);
-    int ept = ept_for((long long)rp.R * rp.h, nt);
-    int ldm = rp.ldm;
-    dim3 grid(rp.Nrho / rp.R, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
-    LP_DISPATCH_EPT(ept, k_rows_inv, grid, dim3(nt), rows_lds_bytes(ldm, rp.R), s, rp, G, wrange, ldm);
+    RowPass rg = rp;
+    rg.R = std::min(rg.R, 8);
+    rg = fit_zgroup(rg, 3LL * (rg.Nrho / rg.R));
+    int nt = threads_for((long long)rg.R * rg.h);
+    int ept = ept_for((long long)rg.R * rg.h, nt);
+    int ldm = rg.ldm;
+    dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
+    LP_DISPATCH_EPT(ept, k_rows_inv, grid, dim3(nt), rows_lds_bytes(ldm, rg.R), s, rg, G, wrange, ldm);
 }
 
 // middle radix of M = 16 * MID * 16, or 0 when the register kernel does not apply
@@ -1298,7 +1316,8 @@ static int cols16_mid(int M) {
     return M / 256;
 }
 
-void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s) {
+void launch_cols(const RowPass& rp0, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s) {
+    const RowPass rp = fit_zgroup(rp0, 3LL * rp0.h);
     if (int mid = rp.fft16 ? cols16_mid(rp.Nrho) : 0) {
         const int T16 = rp.Nrho / 16;
         // column 0 (packed theta-DC / Nyquist pair) on the generic kernel, one

@@ -21,7 +21,7 @@ KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxca
 RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16")
 SPACE = {
     "rows": (4, 8, 16),
-    "zgroup": (1, 2, 4),
+    "zgroup": (1, 2, 4, 8),
     "gzgroup": (1, 2, 4, 8),
     "colchunk": (32, 64, 128),
     "fft16": (1, 0),
