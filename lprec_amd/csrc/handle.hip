@@ -70,7 +70,8 @@ struct lp_handle {
     float* d_tmp = nullptr;   // row-sweep output feeding the column sweep (aliases G when it fits)
     float2* d_S = nullptr;
     std::map<int, float2*> tw;
-    FftPlan plan_h{}, plan_N{}, plan_M{};
+    FftPlan plan_h{}, plan_N{}, plan_M{}, plan_F{};
+    int Lf = 0;   // complex length of the FBP filter transform
 
     float* d_fc = nullptr;
     float2* d_cmapF = nullptr;
@@ -605,7 +606,9 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
     if (any_extra && ((rc = h->upload(&h->d_xrow, xrow)) || (rc = h->upload(&h->d_xent, xent)))) return rc;
     h->nxent = (int)xent.size();
     if (filt) {
-        // kernel of the reference filter: h4 = IFFT_4N(ifftshift(filter)) / 4N; keep lags |l| < N
+        // kernel of the reference filter: h4 = IFFT_4N(ifftshift(filter)) / 4N; keep lags |l| < N.
+        // Only those lags reach the N outputs, so any circular length M2 >= 2N - 1 gives the same
+        // result; M2 = 2N (padding to a power of two measured slower: twice the points per line).
         int M4 = 4 * N, M2 = 2 * N;
         std::vector<std::complex<double>> a(M4);
         for (int k = 0; k < M4; ++k) a[k] = (double)filt[(k + M4 / 2) % M4];
@@ -615,9 +618,16 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         for (int l = -(N - 1); l <= N - 1; ++l) b[(l + M2) % M2] = a[(l + M4) % M4].real() / M4;
         lpfft::Plan fwd2(M2, -1);
         fwd2.exec(b);
-        std::vector<float> H2(N + 1);
-        for (int k = 0; k <= N; ++k) H2[k] = (float)(b[k].real() / (double)M2);
-        if ((rc = h->upload(&h->d_H2, H2)) || (rc = h->twiddles(2 * N))) return rc;
+        const int Lh = M2 / 2;
+        std::vector<float> H2(Lh + 1);
+        for (int k = 0; k <= Lh; ++k) H2[k] = (float)(b[k].real() / (double)M2);
+        try {
+            h->plan_F = make_plan(Lh);
+        } catch (const std::exception& e) {
+            return h->fail(LP_ERR_UNSUPPORTED, e.what());
+        }
+        h->Lf = Lh;
+        if ((rc = h->upload(&h->d_H2, H2)) || (rc = h->twiddles(Lh)) || (rc = h->twiddles(M2))) return rc;
         h->has_filter = true;
     }
     h->has_adj = true;
@@ -706,9 +716,10 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     lpk::SinoPrep sp;
     sp.N0 = h->N0; sp.N = N; sp.P0 = h->P0; sp.P = h->P; sp.os = h->os; sp.lb = N / 2 - h->cor;
     sp.H2 = h->has_filter ? h->d_H2 : nullptr;
-    sp.planN = h->plan_N;
-    sp.twN = h->tw[N];
-    sp.tw2N = h->has_filter ? h->tw[2 * N] : h->tw[N];
+    sp.Lh = h->has_filter ? h->Lf : N;
+    sp.planN = h->has_filter ? h->plan_F : h->plan_N;
+    sp.twN = h->has_filter ? h->tw[h->Lf] : nullptr;
+    sp.tw2N = h->has_filter ? h->tw[2 * h->Lf] : nullptr;
     hipEvent_t* ev = h->prof_begin(1);
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     mark(0);
@@ -848,9 +859,10 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     lpk::SinoPrep sp;
     sp.N0 = h->N0; sp.N = N; sp.P0 = h->P0; sp.P = h->P; sp.os = h->os; sp.lb = N / 2 - h->cor;
     sp.H2 = h->has_filter ? h->d_H2 : nullptr;
-    sp.planN = h->plan_N;
-    sp.twN = h->tw[N];
-    sp.tw2N = h->has_filter ? h->tw[2 * N] : h->tw[N];
+    sp.Lh = h->has_filter ? h->Lf : N;
+    sp.planN = h->has_filter ? h->plan_F : h->plan_N;
+    sp.twN = h->has_filter ? h->tw[h->Lf] : nullptr;
+    sp.tw2N = h->has_filter ? h->tw[2 * h->Lf] : nullptr;
     if (h->cubic) {
         lpk::launch_col_iir_T(acc, h->d_tmp, ns0, h->P, N, h->colchunk, s);
         lpk::launch_sino_rows_T(h->d_tmp, R, ns0, sp, true, s);

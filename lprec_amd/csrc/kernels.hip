@@ -170,6 +170,34 @@ __global__ void k_image_rows(const float* __restrict__ f, float* __restrict__ fc
 // circular convolution of the 4N zero-padded row with a real even kernel; only lags
 // |l| < N reach the N outputs, so it is evaluated exactly as a length-2N circular
 // convolution with the kernel restricted to those lags (spectrum H2, built on host).
+// The FBP filter of one line of N reals (in place): the length-2Lh circular
+// convolution (2Lh >= 2N) with the kernel restricted to lags
+// |l| < N, whose spectrum H2 [Lh+1] is real; the N outputs equal the
+// length-2N circular (and the reference's 4N padded) convolution.  The 2Lh
+// real transform is a complex FFT of length Lh with R2C/C2R post-processing.
+template <int EPT>
+__device__ void filter_line(float2* smc, float* line, const SinoPrep& sp) {
+    const int N = sp.N, Lh = sp.Lh;
+    for (int e = threadIdx.x; e < Lh; e += blockDim.x)
+        smc[lidx(e)] = 2 * e < N ? make_float2(line[2 * e], line[2 * e + 1]) : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft_lds<EPT>(smc, Lh, 1, sp.planN, sp.twN, -1);
+    r2c_post(smc, Lh, 1, Lh, sp.tw2N);
+    for (int k = threadIdx.x; k < Lh; k += blockDim.x) {
+        float2 v = smc[lidx(k)];
+        if (k == 0) smc[lidx(0)] = make_float2(v.x * sp.H2[0], v.y * sp.H2[Lh]);
+        else smc[lidx(k)] = make_float2(v.x * sp.H2[k], v.y * sp.H2[k]);
+    }
+    __syncthreads();
+    c2r_pre(smc, Lh, 1, Lh, sp.tw2N);
+    fft_lds<EPT>(smc, Lh, 1, sp.planN, sp.twN, +1);
+    for (int e = threadIdx.x; e < N / 2; e += blockDim.x) {
+        float2 v = smc[lidx(e)];
+        line[2 * e] = v.x;
+        line[2 * e + 1] = v.y;
+    }
+}
+
 template <int EPT, bool FILT>
 __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc, SinoPrep sp, int cubic) {
     extern __shared__ float2 smc[];
@@ -183,7 +211,7 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
     const int a = ai / sp.os;
     const float* src = R + ((size_t)z * sp.P0 + a) * sp.N0;
     // N + N floats after the FFT buffer (which is only allocated with a filter)
-    float* line = FILT ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
+    float* line = FILT ? reinterpret_cast<float*>(smc + lds_slots(sp.Lh)) : reinterpret_cast<float*>(smc);
     float* tmp = line + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) {
         int xs = min(max(x - sp.lb, 0), sp.N0 - 1);
@@ -191,25 +219,7 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
     }
     if (FILT) {
         __syncthreads();
-        // length-2N real signal [line, 0] as N complex samples
-        for (int e = threadIdx.x; e < N; e += blockDim.x)
-            smc[lidx(e)] = 2 * e < N ? make_float2(line[2 * e], line[2 * e + 1]) : make_float2(0.f, 0.f);
-        __syncthreads();
-        fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, -1);
-        r2c_post(smc, N, 1, N, sp.tw2N);
-        for (int k = threadIdx.x; k < N; k += blockDim.x) {
-            float2 v = smc[lidx(k)];
-            if (k == 0) smc[lidx(0)] = make_float2(v.x * sp.H2[0], v.y * sp.H2[N]);
-            else smc[lidx(k)] = make_float2(v.x * sp.H2[k], v.y * sp.H2[k]);
-        }
-        __syncthreads();
-        c2r_pre(smc, N, 1, N, sp.tw2N);
-        fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, +1);
-        for (int e = threadIdx.x; e < N / 2; e += blockDim.x) {
-            float2 v = smc[lidx(e)];
-            line[2 * e] = v.x;
-            line[2 * e + 1] = v.y;
-        }
+        filter_line<EPT>(smc, line, sp);
     }
     __syncthreads();
     if (cubic) iir_line_lds(line, tmp, N);
@@ -1144,7 +1154,7 @@ int ept_for(long long points, int nt) {
 
 size_t rows_lds_bytes(int ldm, int R) { return (size_t)lds_slots(R * ldm) * sizeof(float2); }
 size_t cols_lds_bytes(int M) { return (size_t)lds_slots(M) * sizeof(float2); }
-size_t sino_lds_bytes(int N) { return (size_t)lds_slots(N) * sizeof(float2) + (size_t)(2 * N) * sizeof(float); }
+size_t sino_lds_bytes(int Lh, int N) { return (size_t)lds_slots(Lh) * sizeof(float2) + (size_t)(2 * N) * sizeof(float); }
 
 // dynamic LDS above 64 KiB (gfx950 has 160 KiB per workgroup) is opted into
 // per kernel; a refusal surfaces as a launch error in check_launch
@@ -1189,14 +1199,15 @@ void launch_image_rows(const float* f, float* fc, int nz, int N0, int N, bool cu
 }
 
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
-    int nt = threads_for(sp.N);
-    int ept = ept_for(sp.N, nt);
     if (!sp.H2) {
         hipLaunchKernelGGL((k_sino_rows<4, false>), dim3(sp.P, nz), dim3(256), 2 * (size_t)sp.N * sizeof(float), s, R,
                            Rc, sp, (int)cubic);
         return;
     }
-    LP_DISPATCH_EPT2(ept, k_sino_rows, true, dim3(sp.P, nz), dim3(nt), sino_lds_bytes(sp.N), s, R, Rc, sp, (int)cubic);
+    int nt = threads_for(sp.Lh);
+    int ept = ept_for(sp.Lh, nt);
+    LP_DISPATCH_EPT2(ept, k_sino_rows, true, dim3(sp.P, nz), dim3(nt), sino_lds_bytes(sp.Lh, sp.N), s, R, Rc, sp,
+                     (int)cubic);
 }
 
 void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s) {
@@ -1488,30 +1499,14 @@ __global__ void k_sino_rows_T(const float* __restrict__ Rc, float* __restrict__ 
     const int a = blockIdx.x, z = blockIdx.y;
     const int ai = a * sp.os;
     const float* src = Rc + ((size_t)z * sp.P + ai) * N;
-    float* line = FILT ? reinterpret_cast<float*>(smc + lds_slots(N)) : reinterpret_cast<float*>(smc);
+    float* line = FILT ? reinterpret_cast<float*>(smc + lds_slots(sp.Lh)) : reinterpret_cast<float*>(smc);
     float* tmp = line + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) line[x] = src[x];
     __syncthreads();
     if (cubic) iir_line_lds_T(line, tmp, N);
     if (FILT) {
-        for (int e = threadIdx.x; e < N; e += blockDim.x)
-            smc[lidx(e)] = 2 * e < N ? make_float2(line[2 * e], line[2 * e + 1]) : make_float2(0.f, 0.f);
-        __syncthreads();
-        fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, -1);
-        r2c_post(smc, N, 1, N, sp.tw2N);
-        for (int k = threadIdx.x; k < N; k += blockDim.x) {
-            float2 v = smc[lidx(k)];
-            if (k == 0) smc[lidx(0)] = make_float2(v.x * sp.H2[0], v.y * sp.H2[N]);
-            else smc[lidx(k)] = make_float2(v.x * sp.H2[k], v.y * sp.H2[k]);
-        }
-        __syncthreads();
-        c2r_pre(smc, N, 1, N, sp.tw2N);
-        fft_lds<EPT>(smc, N, 1, sp.planN, sp.twN, +1);
-        for (int e = threadIdx.x; e < N / 2; e += blockDim.x) {
-            float2 v = smc[lidx(e)];
-            line[2 * e] = v.x;
-            line[2 * e + 1] = v.y;
-        }
+        // the filter is symmetric (a real even spectrum): its own transpose
+        filter_line<EPT>(smc, line, sp);
         __syncthreads();
     }
     // sums of the replicated columns [0, lb) and (rb, N)
@@ -1702,14 +1697,15 @@ void launch_image_rows_T(const float* fc, float* f, int nz, int N0, int N, bool 
 }
 
 void launch_sino_rows_T(const float* Rc, float* R, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
-    int nt = threads_for(sp.N);
-    int ept = ept_for(sp.N, nt);
     if (!sp.H2) {
         hipLaunchKernelGGL((k_sino_rows_T<4, false>), dim3(sp.P0, nz), dim3(256), 2 * (size_t)sp.N * sizeof(float), s,
                            Rc, R, sp, (int)cubic);
         return;
     }
-    LP_DISPATCH_EPT2(ept, k_sino_rows_T, true, dim3(sp.P0, nz), dim3(nt), sino_lds_bytes(sp.N), s, Rc, R, sp, (int)cubic);
+    int nt = threads_for(sp.Lh);
+    int ept = ept_for(sp.Lh, nt);
+    LP_DISPATCH_EPT2(ept, k_sino_rows_T, true, dim3(sp.P0, nz), dim3(nt), sino_lds_bytes(sp.Lh, sp.N), s, Rc, R, sp,
+                     (int)cubic);
 }
 
 }  // namespace lpk

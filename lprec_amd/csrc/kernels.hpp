@@ -25,10 +25,11 @@ void launch_image_rows(const float* f, float* fc, int nz, int N0, int N, bool cu
 // edge-replicate, optional FBP convolution, s-direction prefilter
 struct SinoPrep {
     int N0, N, P0, P, os, lb;
-    const float* H2;          // [N+1] real spectrum of the length-2N filter kernel (or null)
-    FftPlan planN;            // complex FFT of length N (the 2N real transform)
-    const float2* twN;        // W_N^j, j < N
-    const float2* tw2N;       // W_{2N}^k, k <= N/2
+    int Lh;                   // complex length of the filter transform (2 Lh >= 2N real)
+    const float* H2;          // [Lh+1] real spectrum of the length-2Lh filter kernel (or null)
+    FftPlan planN;            // complex FFT of length Lh (the 2Lh real transform)
+    const float2* twN;        // W_Lh^j, j < Lh
+    const float2* tw2N;       // W_{2Lh}^k, k <= Lh/2
 };
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s);
 // y-direction prefilter of [nz][L][W] (columns), chunked with halo, in -> out
@@ -133,7 +134,7 @@ void launch_zero(float* p, size_t n, hipStream_t s);
 
 size_t rows_lds_bytes(int ldm, int R);
 size_t cols_lds_bytes(int M);
-size_t sino_lds_bytes(int N);
+size_t sino_lds_bytes(int Lh, int N);
 int ept_for(long long points, int nt);
 
 }  // namespace lpk
