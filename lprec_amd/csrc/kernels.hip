@@ -165,11 +165,6 @@ __global__ void k_image_rows(const float* __restrict__ f, float* __restrict__ fc
     for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = line[x];
 }
 
-// adj input: copy3Dshifted (lpRgpu.cu:201) + stepangles (:202) + padker (:203) +
-// applyFilter (:333-382) + s-direction prefilter.  The reference's FBP filter is a
-// circular convolution of the 4N zero-padded row with a real even kernel; only lags
-// |l| < N reach the N outputs, so it is evaluated exactly as a length-2N circular
-// convolution with the kernel restricted to those lags (spectrum H2, built on host).
 // The FBP filter of one line of N reals (in place): the length-2Lh circular
 // convolution (2Lh >= 2N) with the kernel restricted to lags
 // |l| < N, whose spectrum H2 [Lh+1] is real; the N outputs equal the
@@ -198,6 +193,11 @@ __device__ void filter_line(float2* smc, float* line, const SinoPrep& sp) {
     }
 }
 
+// adj input: copy3Dshifted (lpRgpu.cu:201) + stepangles (:202) + padker (:203) +
+// applyFilter (:333-382) + s-direction prefilter.  The reference's FBP filter is a
+// circular convolution of the 4N zero-padded row with a real even kernel; only lags
+// |l| < N reach the N outputs, so it is evaluated exactly as a length-2N circular
+// convolution with the kernel restricted to those lags (spectrum H2, built on host).
 template <int EPT, bool FILT>
 __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc, SinoPrep sp, int cubic) {
     extern __shared__ float2 smc[];
