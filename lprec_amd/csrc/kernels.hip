@@ -330,8 +330,9 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // grids): the block stages the source box of its output tile with coalesced
 // row loads, then evaluates every sample of the tile from LDS.  A block owns
 // GROUP tiles that share one output tile (the three spans of the
-// LP -> Cartesian gather, lpRgpu.cu:328, summed in span order); with
-// GROUP = 1 each sample is stored as soon as it is computed.
+// LP -> Cartesian gather, lpRgpu.cu:328, summed in span order); the
+// single-span gathers run k_gather1, which keeps each point's sample
+// geometry across the slices of its group.
 // KIND only names the instance (0 image->LP, 1 sinogram->LP, 2 LP->polar,
 // 3 LP->Cartesian) so that profiles separate the four gathers.
 
@@ -449,6 +450,96 @@ __device__ __forceinline__ float box_sample(const float* box, const GTile& t, fl
     const float* q1 = box + (int)slot;
     const float* q2 = q1 + sy;
     return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
+}
+
+// Single-span gathers with the sample geometry cached: the LDS slot of tap
+// (0,0), the fractional offsets and the output scale of the block's points are
+// computed once and reused for every slice of the block's group (only the
+// staged box changes from slice to slice).
+template <int KIND>
+__global__ void __launch_bounds__(kGatherThreads, 5) k_gather1(Gather g, const GTile* __restrict__ tiles) {
+    extern __shared__ float box[];
+    constexpr int PPT = kGatherPPT;
+    const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int zg = lb % nzg, tile = lb / nzg;
+    const GTile t = tiles[tile];
+    const int lr = threadIdx.x >> t.lnc, lc = threadIdx.x & (t.nc - 1);
+    const int rstep = kGatherThreads >> t.lnc;
+    const unsigned o_lane = (unsigned)(lr * g.o_row + lc) * 4u;
+    const unsigned c_lane = (unsigned)(lr * g.c_row + lc) * 8u;
+    const unsigned s_lane = (unsigned)(lr * g.s_row + lc) * 4u;
+    const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    const float2* cb = g.coord + (size_t)t.span * g.c_span + (size_t)(t.r0 + g.c_ro) * g.c_row + t.c0 + g.c_co;
+    const float* sb = g.scale ? g.scale + (size_t)t.r0 * g.s_row + t.c0 : nullptr;
+    const int sy = t.bw | 1;
+    const int cubic = g.cubic;
+    // box-relative origin of a sample's first tap and the largest valid slot
+    const float ox = (float)(t.x0 + (cubic ? 1 : 0)), oy = (float)(t.y0 + (cubic ? 1 : 0));
+    const float maxslot = cubic ? (float)((t.bh - 4) * sy + t.bw - 4) : (float)((t.bh - 2) * sy + t.bw - 2);
+    int slot[PPT];
+    float ax[PPT], ay[PPT], sc[PPT];
+    float2 c[PPT];
+#pragma unroll
+    for (int e = 0; e < PPT; ++e) {
+        c[e] = make_float2(__int_as_float(0x7fc00000), 0.f);
+        sc[e] = 1.f;
+        if (lr + e * rstep < t.nr) {
+            c[e] = ld_at(cb + (size_t)(e * rstep) * g.c_row, c_lane);
+            if (sb) sc[e] = ld_at(sb + (size_t)(e * rstep) * g.s_row, s_lane);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < PPT; ++e) {
+        const float fx = floorf(c[e].x), fy = floorf(c[e].y);
+        ax[e] = c[e].x - fx;
+        ay[e] = c[e].y - fy;
+        const float sl = fmaf(fy - oy, (float)sy, fx - ox);
+        slot[e] = c[e].x == c[e].x ? (int)__builtin_amdgcn_fmed3f(sl, 0.f, maxslot) : -1;
+    }
+    for (int z = z0; z < z1; ++z) {
+        float* O = g.out + (size_t)z * g.o_z + (size_t)t.span * g.o_span + (size_t)t.r0 * g.o_row + t.c0;
+        if (t.bw > 0) load_box<8>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
+        __syncthreads();
+        // two points per step; the cached values of the step are selected by
+        // uniform branches so that the arrays stay in registers
+#pragma unroll 1
+        for (int e = 0; e < PPT; e += 2) {
+            int sl[2];
+            float fx[2], fy[2], s2[2];
+#pragma unroll
+            for (int q = 0; q < PPT; q += 2)
+                if (q == e) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) { sl[u] = slot[q + u]; fx[u] = ax[q + u]; fy[u] = ay[q + u]; s2[u] = sc[q + u]; }
+                }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (lr + (e + u) * rstep < t.nr) {
+                    float v = 0.f;
+                    if (sl[u] >= 0) {
+                        const float* q = box + sl[u];
+                        if (cubic) {
+                            float wx[4], wy[4];
+                            bspline_w6(fx[u], wx);
+                            bspline_w6(fy[u], wy);
+#pragma unroll
+                            for (int b = 0; b < 4; ++b) {
+                                const float* qr = q + b * sy;
+                                v += wy[b] * (wx[0] * qr[0] + wx[1] * qr[1] + wx[2] * qr[2] + wx[3] * qr[3]);
+                            }
+                            v *= (1.0f / 36.0f);
+                        } else {
+                            const float* q2 = q + sy;
+                            v = (1.f - fy[u]) * ((1.f - fx[u]) * q[0] + fx[u] * q[1]) + fy[u] * ((1.f - fx[u]) * q2[0] + fx[u] * q2[1]);
+                        }
+                    }
+                    st_at(O + (size_t)((e + u) * rstep) * g.o_row, o_lane, v * s2[u]);
+                }
+            }
+        }
+        __syncthreads();
+    }
 }
 
 template <int KIND, int GROUP>
@@ -1226,9 +1317,9 @@ void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hi
     if (kind == 3 && g.group != 3) return;   // the Cartesian gather always sums three spans
     if (kind != 3 && g.group != 1) return;
     switch (kind) {
-        case 0: hipLaunchKernelGGL((k_gather<0, 1>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-        case 1: hipLaunchKernelGGL((k_gather<1, 1>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-        case 2: hipLaunchKernelGGL((k_gather<2, 1>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 0: hipLaunchKernelGGL((k_gather1<0>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 1: hipLaunchKernelGGL((k_gather1<1>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        case 2: hipLaunchKernelGGL((k_gather1<2>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
         default: hipLaunchKernelGGL((k_gather<3, 3>), grid, dim3(kGatherThreads), lds, s, g, tiles); break;
     }
 }
