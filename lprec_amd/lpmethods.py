@@ -114,6 +114,60 @@ def em(lp, init_recon, tomo0, num_iter, reg_par, gpu):
     return _out(recon, as_numpy)
 
 
+class GraphedEM:
+    """EM iterations (the update of em()) with one whole iteration -- fwd,
+    ratio, adj, multiplicative update -- recorded once into a HIP graph
+    (torch.cuda.CUDAGraph) and replayed per iteration.  Every buffer is
+    preallocated and every operation writes in place, so the recorded
+    iteration allocates nothing; the arithmetic and its order are those of
+    em().  ``step(n)`` runs n iterations (eagerly until ``capture()``)."""
+
+    def __init__(self, lp, init_recon, tomo0, reg_par, gpu=0):
+        self.recon, self.tomo, self.as_numpy = _setup(lp, init_recon, tomo0, gpu)
+        self.lp, self.gpu = lp, gpu
+        self.g = torch.zeros_like(self.tomo)
+        self.ratio = torch.zeros_like(self.tomo)
+        self.upd = torch.zeros_like(self.recon)
+        self.xi = torch.zeros_like(self.recon)
+        self.eps = float(np.float32(reg_par))
+        lp.adjp(self.xi, torch.ones_like(self.tomo), gpu)
+        self.xi += 1e-5
+        self.graph = None
+
+    def _iteration(self):
+        self.lp.fwdp(self.g, self.recon, self.gpu)
+        torch.add(self.g, self.eps, out=self.ratio)
+        torch.div(self.tomo, self.ratio, out=self.ratio)
+        self.lp.adjp(self.upd, self.ratio, self.gpu)
+        self.upd.div_(self.xi)
+        self.recon.mul_(self.upd)
+
+    def capture(self):
+        """Record one iteration (recording runs nothing, so the iterate does
+        not advance)."""
+        dev = self.recon.device
+        torch.cuda.synchronize(dev)
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=dev)   # default priority
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph, stream=side):
+                self._iteration()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        self.graph = graph
+        return graph
+
+    def step(self, n=1):
+        for _ in range(n):
+            if self.graph is None:
+                self._iteration()
+            else:
+                self.graph.replay()
+
+    def result(self):
+        return _out(self.recon.clone(), self.as_numpy)
+
+
 def _tv_dual(recon, prox0x, prox0y, c, lam):
     # proximal step of the TV dual (lpmethods.py:158-163)
     prox0x[:, :, :-1] += c * (recon[:, :, 1:] - recon[:, :, :-1])

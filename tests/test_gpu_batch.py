@@ -104,3 +104,37 @@ def test_fbp_full_size_centre_offset_matches_mirror(torch_cuda):
     e = rel(rec.cpu().numpy(), mirror.adj(tab, R))
     print("FBP N=2064 vs mirror: %.2e" % e, flush=True)
     assert e <= 1e-5
+
+
+def test_graphed_em_matches_eager_every_replay(torch_cuda):
+    """One EM iteration captured in a HIP graph and replayed 6 times, with
+    eager work and a synchronisation between replays, against the same
+    iterations run eagerly (and against lpmethods.em)."""
+    torch = torch_cuda
+    from lprec_amd import lpTransform, lpmethods
+    from lprec_amd.phantom import scan
+    N0, P0, ns = 256, 384, 4
+    lp = lpTransform(N0, P0, ns, "none", N0 / 2, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    _, g = scan(ns, N0, P0, 12, seed=5, photons=5e3)
+    tomo = torch.from_numpy(np.maximum(g, 0)).cuda()
+    init = torch.full((ns, N0, N0), 1e-3, device="cuda")
+    graphed = lpmethods.GraphedEM(lp, init, tomo, 1e-3, 0)
+    eager = lpmethods.GraphedEM(lp, init, tomo, 1e-3, 0)
+    graphed.capture()
+    worst = 0.0
+    for it in range(6):
+        graphed.step()
+        torch.cuda.synchronize()
+        eager.step()
+        torch.cuda.synchronize()
+        a, b = graphed.recon.double(), eager.recon.double()
+        err = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        worst = max(worst, err)
+        print("replay %d: relative difference to eager %.2e" % (it + 1, err), flush=True)
+    ref = lpmethods.em(lp, init, tomo, 6, 1e-3, 0)
+    e_ref = float(torch.linalg.norm(graphed.recon.double() - ref.double()) / torch.linalg.norm(ref.double()))
+    print("graphed vs lpmethods.em after 6 iterations: %.2e" % e_ref, flush=True)
+    assert worst < 1e-5
+    assert e_ref < 1e-5
