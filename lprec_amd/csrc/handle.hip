@@ -70,7 +70,7 @@ struct lp_handle {
     float* d_tmp = nullptr;   // row-sweep output feeding the column sweep (aliases G when it fits)
     float2* d_S = nullptr;
     std::map<int, float2*> tw;
-    FftPlan plan_h{}, plan_N{}, plan_M{}, plan_F{};
+    FftPlan plan_h{}, plan_N{}, plan_M{}, plan_F{}, plan_2N{};
     int Lf = 0;   // complex length of the FBP filter transform
 
     float* d_fc = nullptr;
@@ -623,11 +623,14 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         for (int k = 0; k <= Lh; ++k) H2[k] = (float)(b[k].real() / (double)M2);
         try {
             h->plan_F = make_plan(Lh);
+            h->plan_2N = make_plan(2 * N);
         } catch (const std::exception& e) {
             return h->fail(LP_ERR_UNSUPPORTED, e.what());
         }
         h->Lf = Lh;
-        if ((rc = h->upload(&h->d_H2, H2)) || (rc = h->twiddles(Lh)) || (rc = h->twiddles(M2))) return rc;
+        if ((rc = h->upload(&h->d_H2, H2)) || (rc = h->twiddles(Lh)) || (rc = h->twiddles(M2)) ||
+            (rc = h->twiddles(2 * N)))
+            return rc;
         h->has_filter = true;
     }
     h->has_adj = true;
@@ -720,6 +723,8 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     sp.planN = h->has_filter ? h->plan_F : h->plan_N;
     sp.twN = h->has_filter ? h->tw[h->Lf] : nullptr;
     sp.tw2N = h->has_filter ? h->tw[2 * h->Lf] : nullptr;
+    sp.plan2N = h->plan_2N;
+    sp.tw2Nc = h->has_filter ? h->tw[2 * N] : nullptr;
     hipEvent_t* ev = h->prof_begin(1);
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     mark(0);
@@ -863,6 +868,8 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     sp.planN = h->has_filter ? h->plan_F : h->plan_N;
     sp.twN = h->has_filter ? h->tw[h->Lf] : nullptr;
     sp.tw2N = h->has_filter ? h->tw[2 * h->Lf] : nullptr;
+    sp.plan2N = h->plan_2N;
+    sp.tw2Nc = h->has_filter ? h->tw[2 * N] : nullptr;
     if (h->cubic) {
         lpk::launch_col_iir_T(acc, h->d_tmp, ns0, h->P, N, h->colchunk, s);
         lpk::launch_sino_rows_T(h->d_tmp, R, ns0, sp, true, s);

@@ -105,13 +105,14 @@ __device__ __forceinline__ void stage_in(int n, Load load, Store store) {
 // main_kernels.cuh:97-142).  Each thread owns a chunk and restarts the causal and
 // anticausal recursions kHalo samples outside it (truncation |z|^17 < 2e-10), except
 // at the line ends where the reference's own initialisations are used.
-__device__ void iir_line_lds(float* line, float* tmp, int L) {
+// (tid, nt: this thread's index among the nt threads that share the line;
+// every thread of the block calls it, since it synchronises the block)
+__device__ void iir_line_lds(float* line, float* tmp, int L, int tid, int nt) {
     const float z = kPole;
-    const int nt = blockDim.x;
     // odd chunk length: lanes start T floats apart, so an odd T keeps the
     // lanes of a wave on distinct banks
     const int T = ((L + nt - 1) / nt) | 1;
-    const int s = threadIdx.x * T;
+    const int s = tid * T;
     const int e = min(s + T, L);
     if (s < L) {
         float c;
@@ -160,7 +161,7 @@ __global__ void k_image_rows(const float* __restrict__ f, float* __restrict__ fc
         line[x] = (rowin && xs >= 0 && xs < N0) ? src[xs] : 0.f;
     }
     __syncthreads();
-    if (cubic) iir_line_lds(line, tmp, N);
+    if (cubic) iir_line_lds(line, tmp, N, threadIdx.x, blockDim.x);
     float* dst = fc + ((size_t)z * N + yi) * N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = line[x];
 }
@@ -222,8 +223,60 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
         filter_line<EPT>(smc, line, sp);
     }
     __syncthreads();
-    if (cubic) iir_line_lds(line, tmp, N);
+    if (cubic) iir_line_lds(line, tmp, N, threadIdx.x, blockDim.x);
     for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = line[x];
+}
+
+// The same preparation for two kept angles per block when a filter is set: the
+// kernel's spectrum is real and even, so the complex line x0 + i x1 filtered
+// by one length-2N complex FFT pair gives the two filtered lines as its real
+// and imaginary parts (no R2C/C2R steps, half the FFT launches per line).
+// The block also writes the zero (stuffed) rows that follow each angle.
+template <int EPT>
+__global__ void k_sino_rows_pair(const float* __restrict__ R, float* __restrict__ Rc, SinoPrep sp, int cubic) {
+    extern __shared__ float2 smc[];
+    const int N = sp.N, M = 2 * N;
+    const int a0 = 2 * blockIdx.x, z = blockIdx.y;
+    const int nrow = min(2, sp.P0 - a0);
+    float* line = reinterpret_cast<float*>(smc + lds_slots(M));   // [2][N]
+    float* tmp = line + 2 * N;                                     // [2][N]
+    for (int u = 0; u < nrow; ++u)
+        for (int j = 1; j < sp.os; ++j) {
+            float* dz = Rc + ((size_t)z * sp.P + (size_t)(a0 + u) * sp.os + j) * N;
+            for (int x = threadIdx.x; x < N; x += blockDim.x) dz[x] = 0.f;
+        }
+    for (int u = 0; u < 2; ++u) {
+        const float* src = R + ((size_t)z * sp.P0 + min(a0 + u, sp.P0 - 1)) * sp.N0;
+        for (int x = threadIdx.x; x < N; x += blockDim.x) {
+            int xs = min(max(x - sp.lb, 0), sp.N0 - 1);
+            line[u * N + x] = u < nrow ? src[xs] : 0.f;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < M; e += blockDim.x)
+        smc[lidx(e)] = e < N ? make_float2(line[e], line[N + e]) : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft_lds<EPT>(smc, M, 1, sp.plan2N, sp.tw2Nc, -1);
+    for (int k = threadIdx.x; k < M; k += blockDim.x) {
+        const float hk = sp.H2[k <= N ? k : M - k];
+        float2 v = smc[lidx(k)];
+        smc[lidx(k)] = make_float2(v.x * hk, v.y * hk);
+    }
+    __syncthreads();
+    fft_lds<EPT>(smc, M, 1, sp.plan2N, sp.tw2Nc, +1);
+    for (int e = threadIdx.x; e < N; e += blockDim.x) {
+        float2 v = smc[lidx(e)];
+        line[e] = v.x;
+        line[N + e] = v.y;
+    }
+    __syncthreads();
+    // the two lines' prefilters run side by side on the two halves of the block
+    const int half = blockDim.x / 2, u = threadIdx.x < half ? 0 : 1;
+    if (cubic) iir_line_lds(line + u * N, tmp + u * N, N, threadIdx.x - u * half, half);
+    for (int v = 0; v < nrow; ++v) {
+        float* dst = Rc + ((size_t)z * sp.P + (size_t)(a0 + v) * sp.os) * N;
+        for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = line[v * N + x];
+    }
 }
 
 // y-direction prefilter (SamplesToCoefficients2DY after the transposes of
@@ -1295,10 +1348,11 @@ void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, boo
                            Rc, sp, (int)cubic);
         return;
     }
-    int nt = threads_for(sp.Lh);
-    int ept = ept_for(sp.Lh, nt);
-    LP_DISPATCH_EPT2(ept, k_sino_rows, true, dim3(sp.P, nz), dim3(nt), sino_lds_bytes(sp.Lh, sp.N), s, R, Rc, sp,
-                     (int)cubic);
+    // two kept angles per block, one complex FFT pair of length 2N
+    int nt = threads_for(2LL * sp.N);
+    int ept = ept_for(2LL * sp.N, nt);
+    size_t lds = (size_t)lds_slots(2 * sp.N) * sizeof(float2) + (size_t)(4 * sp.N) * sizeof(float);
+    LP_DISPATCH_EPT(ept, k_sino_rows_pair, dim3((sp.P0 + 1) / 2, nz), dim3(nt), lds, s, R, Rc, sp, (int)cubic);
 }
 
 void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s) {

@@ -30,6 +30,8 @@ struct SinoPrep {
     FftPlan planN;            // complex FFT of length Lh (the 2Lh real transform)
     const float2* twN;        // W_Lh^j, j < Lh
     const float2* tw2N;       // W_{2Lh}^k, k <= Lh/2
+    FftPlan plan2N;           // complex FFT of length 2N (two filtered lines packed as x0 + i x1)
+    const float2* tw2Nc;      // W_{2N}^j, j < 2N
 };
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s);
 // y-direction prefilter of [nz][L][W] (columns), chunked with halo, in -> out
