@@ -58,3 +58,29 @@ def test_fwd_adj_match_mirror(torch_cuda, N0, P0, ns, filt, cor, interp):
     ef, ea = rel(Rout.cpu().numpy(), Rm), rel(fout.cpu().numpy(), fm)
     print("rel fwd %.3e adj %.3e" % (ef, ea), flush=True)
     assert ef <= TOL and ea <= TOL
+
+
+@pytest.mark.gpu
+def test_bubbles_fixture_backprojection(torch_cuda):
+    """The reference's bubbles fixture (tests/data/Rbub, fbub: Hamming-filtered
+    Radon data of three discs, N = 256, 64 angles): the device back-projection
+    matches the oracle to 1e-5 and the filtered image to the oracle's own
+    fitted error (0.217, angular undersampling)."""
+    import os
+    torch = torch_cuda
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    N, P0 = 256, 64
+    R = np.fromfile(os.path.join(golden, "Rbub"), dtype=np.float32).reshape(N, P0).T.copy()
+    f = np.fromfile(os.path.join(golden, "fbub"), dtype=np.float32).reshape(N, N)
+    lp, tab = make(N, P0, 1, "none", N / 2, "cubic")
+    out = torch.zeros((1, N, N), dtype=torch.float32, device="cuda")
+    lp.adjp(out, torch.from_numpy(R[None]).cuda(), 0)
+    torch.cuda.synchronize()
+    x = out.cpu().numpy()[0].astype(np.float64)
+    xm = mirror.adj(tab, R[None])[0]
+    ref = f[::-1]
+    a = float(np.sum(x * ref) / np.sum(x * x))
+    err = float(np.linalg.norm(a * x - ref) / np.linalg.norm(ref))
+    print("bubbles: rel to oracle %.3e, fitted error to the image %.4f" % (rel(x, xm), err), flush=True)
+    assert rel(x, xm) <= TOL
+    assert err < 0.25

@@ -238,3 +238,35 @@ def test_product_sinogram_generator_matches_oracle_radon():
     # cor = 61 -> padded N = 144, detector column x at s = -1 + 2 (x + 72 - 61) / 143
     s = -1 + 2 * (np.arange(128) + 72 - 61) / 143
     assert np.allclose(ellipse_sinogram(ell, 128, 90, cor=61), ellipse_radon(ell, 128, 90, s=s), atol=1e-12)
+
+
+def _bubbles():
+    # tests/data/fbub, Rbub of the reference (make_bubles.m:1-19): three discs
+    # at N = 256, 64 angles; Rbub holds the exactly Hamming-filtered Radon data
+    # written column-major as [angle][s] (so [s][angle] in C order), fbub the
+    # exactly filtered image
+    N, P0 = 256, 64
+    R = np.fromfile(os.path.join(GOLDEN, "Rbub"), dtype=np.float32).reshape(N, P0).T.copy()
+    f = np.fromfile(os.path.join(GOLDEN, "fbub"), dtype=np.float32).reshape(N, N)
+    return N, P0, R, f
+
+
+def _fitted_error(x, ref):
+    a = float(np.sum(x * ref) / np.sum(x * x))
+    return float(np.linalg.norm(a * x - ref) / np.linalg.norm(ref))
+
+
+def test_reference_bubbles_fixture_backprojection():
+    # the unfiltered back-projection of the pre-filtered data against the
+    # filtered image, up to the fixture's own scale: no reference test states
+    # a tolerance; measured 0.217 (64 angles undersample N = 256).  The image
+    # axis convention is pinned too: the data only match the image flipped
+    # along its rows, every other orientation is far off.
+    N, P0, R, f = _bubbles()
+    pre = pc.precompute(N, P0, 1, N / 2, "cubic", "none", fwd=False)
+    tab = mirror.Tables(pre["glpars"], None, None, pre["adjparsi"], pre["adjparamsf"])
+    x = mirror.adj(tab, R[None].astype(np.float64))[0]
+    err = _fitted_error(x, f[::-1])
+    others = [_fitted_error(x, g) for g in (f, f.T, f[:, ::-1], f[::-1, ::-1])]
+    assert err < 0.25, err
+    assert min(others) > 0.5, others
