@@ -125,20 +125,20 @@ def pmc_summary(root, copy_mib):
 # recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
 MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536916543, 536870912),
-    "fwd.col_iir": (1025694171, 536870912),
-    "fwd.gather_lp": (4428803155, 1133801728),
-    "fwd.rows_fwd": (1092990785, 3744383840),
-    "fwd.cols": (3258449999, 3221741632),
-    "fwd.rows_inv": (3329652687, 1624334832),
-    "fwd.polar_out": (4666527075, 393216000),
-    "adj.sino_rows": (393255564, 786432000),
-    "adj.col_iir": (1519977152, 786432000),
-    "adj.gather_lp": (4048013579, 1610861568),
-    "adj.rows_fwd": (1613190502, 3736210240),
-    "adj.cols": (3259519045, 3221741632),
-    "adj.rows_inv": (3351653621, 1085025376),
-    "adj.cart_out": (3729863959, 536870912),
+    "fwd.prep_rows": (536918691, 536870912),
+    "fwd.col_iir": (1026175509, 536870912),
+    "fwd.gather_lp": (4402107407, 1134007968),
+    "fwd.rows_fwd": (1094833211, 3221225472),
+    "fwd.cols": (3268676619, 3223289984),
+    "fwd.rows_inv": (3222256397, 1623806208),
+    "fwd.polar_out": (4603617899, 393216000),
+    "adj.sino_rows": (393254998, 786432000),
+    "adj.col_iir": (1522345860, 786432000),
+    "adj.gather_lp": (3762465105, 1610872496),
+    "adj.rows_fwd": (1617530984, 3221225472),
+    "adj.cols": (3270321531, 3223289856),
+    "adj.rows_inv": (3222230286, 1084827984),
+    "adj.cart_out": (4601513344, 536870912),
 }
 
 

@@ -597,10 +597,11 @@ __global__ void __launch_bounds__(kGatherThreads, 5) k_gather1(Gather g, const G
 
 template <int KIND, int GROUP>
 __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile* __restrict__ tiles) {
+    static_assert(GROUP > 1, "single-span gathers run k_gather1");
     extern __shared__ float box[];
     constexpr int PPT = kGatherPPT;
-    // coordinates (and scales) run GP points ahead of the sampling; the first
-    // GP are issued before the box load and arrive while it streams in
+    // coordinates run GP points ahead of the sampling; the first GP are
+    // issued before the box load and arrive while it streams in
     constexpr int GP = kGatherPipe;
     static_assert(PPT % GP == 0, "points per thread come in groups of GP");
     // logical order: the slice groups of one tile are neighbours (they share
@@ -622,9 +623,9 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
     const float* sb = g.scale ? g.scale + (size_t)t0.r0 * g.s_row + t0.c0 : nullptr;
     for (int z = z0; z < z1; ++z) {
         float* O = g.out + (size_t)z * g.o_z + (size_t)t0.span * g.o_span + (size_t)t0.r0 * g.o_row + t0.c0;
-        float acc[GROUP > 1 ? PPT : 1];
+        float acc[PPT];
 #pragma unroll
-        for (int e = 0; e < (GROUP > 1 ? PPT : 1); ++e) acc[e] = 0.f;
+        for (int e = 0; e < PPT; ++e) acc[e] = 0.f;
 #pragma unroll 1
         for (int gi = 0; gi < GROUP; ++gi) {
             const GTile t = tb[gi];
@@ -634,59 +635,46 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
                 if (lr + e * rstep < t0.nr) c = ld_at(cb + (size_t)(e * rstep) * g.c_row, c_lane);
                 return c;
             };
-            auto scale_of = [&](int e) {
-                float sc = 1.f;
-                if (GROUP == 1 && sb && lr + e * rstep < t0.nr) sc = ld_at(sb + (size_t)(e * rstep) * g.s_row, s_lane);
-                return sc;
-            };
             float2 cur[GP];
-            float scur[GP];
 #pragma unroll
-            for (int u = 0; u < GP; ++u) { cur[u] = coord_of(u); scur[u] = scale_of(u); }
+            for (int u = 0; u < GP; ++u) cur[u] = coord_of(u);
             if (t.bw > 0) load_box<8>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
             __syncthreads();
 #pragma unroll 1
             for (int e = 0; e < PPT; e += GP) {
                 float2 nxt[GP];
-                float snxt[GP];
 #pragma unroll
-                for (int u = 0; u < GP; ++u) { nxt[u] = cur[u]; snxt[u] = scur[u]; }
+                for (int u = 0; u < GP; ++u) nxt[u] = cur[u];
                 if (e + GP < PPT) {
 #pragma unroll
-                    for (int u = 0; u < GP; ++u) { nxt[u] = coord_of(e + GP + u); snxt[u] = scale_of(e + GP + u); }
+                    for (int u = 0; u < GP; ++u) nxt[u] = coord_of(e + GP + u);
                 }
                 float vals[GP];
 #pragma unroll
                 for (int u = 0; u < GP; ++u) {
                     const bool in = lr + (e + u) * rstep < t0.nr;
                     const float2 c = cur[u];
-                    float v = (in && c.x == c.x) ? box_sample(box, t, c, g.cubic) : 0.f;
-                    vals[u] = v;
-                    if (GROUP == 1 && in) st_at(O + (size_t)((e + u) * rstep) * g.o_row, o_lane, v * scur[u]);
+                    vals[u] = (in && c.x == c.x) ? box_sample(box, t, c, g.cubic) : 0.f;
                 }
-                if constexpr (GROUP > 1) {
-                    // acc is indexed by a loop variable: keep it in registers by
-                    // selecting the group explicitly
+                // acc is indexed by a loop variable: keep it in registers by
+                // selecting the group explicitly
 #pragma unroll
-                    for (int q = 0; q < PPT; q += GP)
-                        if (q == e) {
+                for (int q = 0; q < PPT; q += GP)
+                    if (q == e) {
 #pragma unroll
-                            for (int u = 0; u < GP; ++u) acc[q + u] += vals[u];
-                        }
-                }
+                        for (int u = 0; u < GP; ++u) acc[q + u] += vals[u];
+                    }
 #pragma unroll
-                for (int u = 0; u < GP; ++u) { cur[u] = nxt[u]; scur[u] = snxt[u]; }
+                for (int u = 0; u < GP; ++u) cur[u] = nxt[u];
             }
             __syncthreads();
         }
-        if constexpr (GROUP > 1) {
 #pragma unroll
-            for (int e = 0; e < PPT; ++e) {
-                if (lr + e * rstep < t0.nr) {
-                    float v = acc[e];
-                    if (sb) v *= ld_at(sb + (size_t)(e * rstep) * g.s_row, s_lane);
-                    st_at(O + (size_t)(e * rstep) * g.o_row, o_lane, v);
-                }
+        for (int e = 0; e < PPT; ++e) {
+            if (lr + e * rstep < t0.nr) {
+                float v = acc[e];
+                if (sb) v *= ld_at(sb + (size_t)(e * rstep) * g.s_row, s_lane);
+                st_at(O + (size_t)(e * rstep) * g.o_row, o_lane, v);
             }
         }
     }
