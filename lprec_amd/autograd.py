@@ -127,7 +127,8 @@ def adam_tv(lp, tomo, num_iter=100, reg_par=1e-3, lr=0.1, init="sd", gpu=0, call
         loss = (r * r).sum() + reg_par * tv_norm(f)
         loss.backward()
         opt.step()
-        losses.append(float(loss.detach()))
+        # kept on the device: no host synchronisation per step
+        losses.append(loss.detach())
         if callback is not None:
-            callback(it, f.detach(), losses[-1])
-    return f.detach(), losses
+            callback(it, f.detach(), float(losses[-1]))
+    return f.detach(), [float(v) for v in losses]
