@@ -667,7 +667,7 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     lpk::launch_image_rows(f, h->cubic ? h->d_tmp : h->d_fc, ns0, h->N0, N, h->cubic, s);
     mark(1);
     if (h->stop_after == 1) return check_launch(h, "lp_fwd launch");
-    if (h->cubic) lpk::launch_col_iir(h->d_tmp, h->d_fc, ns0, N, N, h->colchunk, s);
+    if (h->cubic) lpk::launch_col_iir(h->d_tmp, h->d_fc, ns0, N, N, h->colchunk, 1, s);
     mark(2);
     if (h->stop_after == 2) return check_launch(h, "lp_fwd launch");
     lpk::RowPass rp = row_pass(h, ns0);
@@ -731,7 +731,7 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     lpk::launch_sino_rows(R, h->cubic ? h->d_tmp : h->d_Rc, ns0, sp, h->cubic, s);
     mark(1);
     if (h->stop_after == 1) return check_launch(h, "lp_adj launch");
-    if (h->cubic) lpk::launch_col_iir(h->d_tmp, h->d_Rc, ns0, h->P, N, h->colchunk, s);
+    if (h->cubic) lpk::launch_col_iir(h->d_tmp, h->d_Rc, ns0, h->P, N, h->colchunk, h->os, s);
     mark(2);
     if (h->stop_after == 2) return check_launch(h, "lp_adj launch");
     lpk::RowPass rp = row_pass(h, ns0);

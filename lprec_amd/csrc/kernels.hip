@@ -205,8 +205,9 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
     const int N = sp.N;
     const int ai = blockIdx.x, z = blockIdx.y;
     float* dst = Rc + ((size_t)z * sp.P + ai) * N;
-    if (ai % sp.os != 0) {
-        for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = 0.f;
+    if (ai % sp.os != 0) {   // stuffed row; the cubic column sweep does not read it
+        if (!cubic)
+            for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = 0.f;
         return;
     }
     const int a = ai / sp.os;
@@ -231,7 +232,8 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
 // kernel's spectrum is real and even, so the complex line x0 + i x1 filtered
 // by one length-2N complex FFT pair gives the two filtered lines as its real
 // and imaginary parts (no R2C/C2R steps, half the FFT launches per line).
-// The block also writes the zero (stuffed) rows that follow each angle.
+// Without the cubic prefilter the block also writes the zero (stuffed) rows
+// that follow each angle; the cubic column sweep does not read them.
 template <int EPT>
 __global__ void k_sino_rows_pair(const float* __restrict__ R, float* __restrict__ Rc, SinoPrep sp, int cubic) {
     extern __shared__ float2 smc[];
@@ -240,7 +242,7 @@ __global__ void k_sino_rows_pair(const float* __restrict__ R, float* __restrict_
     const int nrow = min(2, sp.P0 - a0);
     float* line = reinterpret_cast<float*>(smc + lds_slots(M));   // [2][N]
     float* tmp = line + 2 * N;                                     // [2][N]
-    for (int u = 0; u < nrow; ++u)
+    for (int u = 0; u < (cubic ? 0 : nrow); ++u)
         for (int j = 1; j < sp.os; ++j) {
             float* dz = Rc + ((size_t)z * sp.P + (size_t)(a0 + u) * sp.os + j) * N;
             for (int x = threadIdx.x; x < N; x += blockDim.x) dz[x] = 0.f;
@@ -283,8 +285,10 @@ __global__ void k_sino_rows_pair(const float* __restrict__ R, float* __restrict_
 // prefilter2D, lpRgpu.cu:229-239): one thread per column, T rows per chunk with a
 // kHalo-row halo on each side held in registers.
 // Out of place: chunks read halos that neighbouring chunks write.
+// stuff > 1: only rows that are multiples of stuff hold data (the angle
+// stuffing of the sinogram); the others are zero and are not read.
 template <int T>
-__global__ void k_col_iir(const float* __restrict__ in, float* __restrict__ outp, int L, int W) {
+__global__ void k_col_iir(const float* __restrict__ in, float* __restrict__ outp, int L, int W, int stuff) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int r0 = blockIdx.y * T;
     const int z = blockIdx.z;
@@ -296,8 +300,8 @@ __global__ void k_col_iir(const float* __restrict__ in, float* __restrict__ outp
     float v[NV];
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
-        int row = r0 - K + n;
-        v[n] = (row >= 0 && row < L) ? col[(size_t)row * W] : 0.f;
+        int row = r0 - K + n;   // uniform over the block
+        v[n] = (row >= 0 && row < L && row % stuff == 0) ? col[(size_t)row * W] : 0.f;
     }
     const float zp = kPole;
     if (r0 == 0) {
@@ -1343,12 +1347,12 @@ void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, boo
     LP_DISPATCH_EPT(ept, k_sino_rows_pair, dim3((sp.P0 + 1) / 2, nz), dim3(nt), lds, s, R, Rc, sp, (int)cubic);
 }
 
-void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s) {
+void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, int stuff, hipStream_t s) {
     dim3 block(256);
     switch (chunk) {
-        case 32: hipLaunchKernelGGL(k_col_iir<32>, dim3((W + 255) / 256, (L + 31) / 32, nz), block, 0, s, in, out, L, W); break;
-        case 128: hipLaunchKernelGGL(k_col_iir<128>, dim3((W + 255) / 256, (L + 127) / 128, nz), block, 0, s, in, out, L, W); break;
-        default: hipLaunchKernelGGL(k_col_iir<64>, dim3((W + 255) / 256, (L + 63) / 64, nz), block, 0, s, in, out, L, W); break;
+        case 32: hipLaunchKernelGGL(k_col_iir<32>, dim3((W + 255) / 256, (L + 31) / 32, nz), block, 0, s, in, out, L, W, stuff); break;
+        case 128: hipLaunchKernelGGL(k_col_iir<128>, dim3((W + 255) / 256, (L + 127) / 128, nz), block, 0, s, in, out, L, W, stuff); break;
+        default: hipLaunchKernelGGL(k_col_iir<64>, dim3((W + 255) / 256, (L + 63) / 64, nz), block, 0, s, in, out, L, W, stuff); break;
     }
 }
 

@@ -35,8 +35,9 @@ struct SinoPrep {
 };
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s);
 // y-direction prefilter of [nz][L][W] (columns), chunked with halo, in -> out
-// (out of place: the chunks overlap through their halos)
-void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, hipStream_t s);
+// (out of place: the chunks overlap through their halos); with stuff > 1 the
+// input rows that are not multiples of stuff are zero and are not read
+void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk, int stuff, hipStream_t s);
 
 // ---- log-polar convolution passes ------------------------------------------
 struct RowPass {
