@@ -40,8 +40,10 @@ def stage_bytes(g):
         "polar_out": 3 * L // 2 + 4 * N0 * P0,     # half of each G (|theta| < beta/2) read, R written
     }
     adj = {
-        "sino_rows": 4 * N0 * P0 + 4 * N * P,
-        "col_iir": 8 * N * P,
+        # the cubic path writes only the kept angle rows of the stuffed
+        # sinogram (P0 of P = os P0); the column sweep reads them and writes all P
+        "sino_rows": 4 * N0 * P0 + 4 * N * P0,
+        "col_iir": 4 * N * P0 + 4 * N * P,
         "gather_lp": 4 * N * P + 3 * L,
         "rows_fwd": 3 * 2 * L,
         "cols": 3 * 2 * L,
@@ -52,11 +54,11 @@ def stage_bytes(g):
 
 
 def pipeline_bytes(g):
-    """B_alg per slice of fwd+adj (compulsory traffic of the fused design):
-    B_fwd = 4N0^2 + 16N^2 + 12L + 4N0P0, B_adj = 4N0P0 + 16NP + 12L + 4N0^2."""
+    """B_alg per slice of fwd+adj (compulsory traffic of the fused design, cubic):
+    B_fwd = 4N0^2 + 16N^2 + 12L + 4N0P0, B_adj = 4N0P0 + 8NP0 + 8NP + 12L + 4N0^2."""
     N0, N, P0, P = g["N0"], g["N"], g["Nproj0"], g["Nproj"]
     L = 4 * g["Nrho"] * g["Ntheta"]
-    return (4 * N0 * N0 + 16 * N * N + 12 * L + 4 * N0 * P0) + (4 * N0 * P0 + 16 * N * P + 12 * L + 4 * N0 * N0)
+    return (4 * N0 * N0 + 16 * N * N + 12 * L + 4 * N0 * P0) + (4 * N0 * P0 + 8 * N * P0 + 8 * N * P + 12 * L + 4 * N0 * N0)
 
 
 # Kernel name prefix -> stage of the pipeline (one fwd or adj call launches
