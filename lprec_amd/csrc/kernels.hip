@@ -509,6 +509,30 @@ __device__ __forceinline__ float box_sample(const float* box, const GTile& t, fl
     return (1.f - ay) * ((1.f - ax) * q1[0] + ax * q1[1]) + ay * ((1.f - ax) * q2[0] + ax * q2[1]);
 }
 
+// Stage the box of tile t like load_box, but, when the padded box (rows of
+// stride bs = bw | 1 including the pad column) lies inside the grid, with
+// loads that write LDS directly, 4 bytes per lane: 64 consecutive box floats
+// are 64 consecutive LDS floats, no registers or LDS stores in between.  The
+// last chunk's lanes past the box reload its origin into LDS floats below the
+// next multiple of 64 (the launch rounds the box allocation up to that).
+__device__ __forceinline__ void stage_box(float* box, const float* __restrict__ S, const GTile& t, int W, int H) {
+    const int bs = t.bw | 1, nb = bs * t.bh;
+    if (t.x0 >= 0 && t.y0 >= 0 && t.x0 + bs <= W && t.y0 + t.bh <= H) {
+        const float* Sb = S + (size_t)t.y0 * W + t.x0;
+        const float inv_bs = 1.0f / (float)bs;
+        const int lane = threadIdx.x & 63;
+        for (int k = (threadIdx.x >> 6) * 64; k < nb; k += kGatherThreads) {
+            const int i = k + lane;
+            const int yy = (int)(((float)i + 0.5f) * inv_bs);
+            const float* src = Sb + (i < nb ? yy * W + (i - yy * bs) : 0);
+            __builtin_amdgcn_global_load_lds(src, box + k, 4, 0, 0);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
+        load_box<8>(box, S, t, W, H);
+    }
+}
+
 // Single-span gathers with the sample geometry cached: the LDS slot of tap
 // (0,0), the fractional offsets and the output scale of the block's points are
 // computed once and reused for every slice of the block's group (only the
@@ -556,7 +580,14 @@ __global__ void __launch_bounds__(kGatherThreads, 5) k_gather1(Gather g, const G
     }
     for (int z = z0; z < z1; ++z) {
         float* O = g.out + (size_t)z * g.o_z + (size_t)t.span * g.o_span + (size_t)t.r0 * g.o_row + t.c0;
-        if (t.bw > 0) load_box<8>(box, g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span, t, g.W, g.H);
+        if (t.bw > 0) {
+            const float* S = g.src + (size_t)z * g.src_z + (size_t)t.span * g.src_span;
+            // the gathers into the LP grids (from the image or the sinogram)
+            // stage directly; the polar output's boxes measured faster
+            // through registers
+            if (KIND == 2) load_box<8>(box, S, t, g.W, g.H);
+            else stage_box(box, S, t, g.W, g.H);
+        }
         __syncthreads();
         // two points per step; the cached values of the step are selected by
         // uniform branches so that the arrays stay in registers
@@ -1359,7 +1390,8 @@ void launch_col_iir(const float* in, float* out, int nz, int L, int W, int chunk
 void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
     if (ntiles <= 0) return;
     dim3 grid((ntiles / g.group) * ((g.nz + g.zgroup - 1) / g.zgroup));
-    size_t lds = (size_t)g.box_floats * sizeof(float);
+    // whole 64-float chunks for stage_box
+    size_t lds = (size_t)((g.box_floats + 63) & ~63) * sizeof(float);
     if (kind == 3 && g.group != 3) return;   // the Cartesian gather always sums three spans
     if (kind != 3 && g.group != 1) return;
     switch (kind) {
@@ -1803,7 +1835,8 @@ void launch_zero(float* p, size_t n, hipStream_t s) {
 void launch_scatter(const Scatter& g, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
     if (ntiles <= 0) return;
     dim3 grid((ntiles / g.group) * ((g.nz + g.zgroup - 1) / g.zgroup));
-    size_t lds = (size_t)g.box_floats * sizeof(float);
+    // whole 64-float chunks for stage_box
+    size_t lds = (size_t)((g.box_floats + 63) & ~63) * sizeof(float);
     switch (kind) {
         case 0: hipLaunchKernelGGL(k_scatter<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
         case 1: hipLaunchKernelGGL(k_scatter<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
