@@ -25,6 +25,18 @@ __device__ __forceinline__ int lidx(int x) { return x + (x >> 4); }
 // complex slots needed for n elements
 __host__ __device__ constexpr int lds_slots(int n) { return n + (n >> 4) + 1; }
 
+// Quotient of 0 <= t < 2^22 by 0 < d < 2^24 without an integer division:
+// (t + 0.5) / d lies at least 0.5 / d from an integer, and the float product
+// (t + 0.5) * fl(1 / d) errs by at most ((t + 0.5) / d) 2^-23, less than that
+// distance while t + 0.5 < 2^22.
+struct FDiv {
+    int d;
+    float inv;
+    __device__ __forceinline__ explicit FDiv(int d_) : d(d_), inv(1.0f / (float)d_) {}
+    __device__ __forceinline__ int quo(int t) const { return (int)(((float)t + 0.5f) * inv); }
+    __device__ __forceinline__ int rem(int t) const { return t - quo(t) * d; }
+};
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -121,12 +133,13 @@ __device__ void fft_stage(float2* buf, int ldm, int nlines, int M, int Ns, const
     const int nbf = M / R;
     const int total = nlines * nbf;
     const int nt = blockDim.x;
+    const FDiv dnbf(nbf), dNs(Ns);
     float2 v[MAXB][R];
 #pragma unroll
     for (int c = 0; c < MAXB; ++c) {
         int t = threadIdx.x + c * nt;
         if (t < total) {
-            int b = t / nbf, i = t - b * nbf;
+            int b = dnbf.quo(t), i = t - b * nbf;
             const int base = b * ldm + i;
 #pragma unroll
             for (int q = 0; q < R; ++q) v[c][q] = buf[lidx(base + q * nbf)];
@@ -138,8 +151,8 @@ __device__ void fft_stage(float2* buf, int ldm, int nlines, int M, int Ns, const
     for (int c = 0; c < MAXB; ++c) {
         int t = threadIdx.x + c * nt;
         if (t < total) {
-            int b = t / nbf, i = t - b * nbf;
-            int k = i % Ns;
+            int b = dnbf.quo(t), i = t - b * nbf;
+            int k = dNs.rem(i);
             if (Ns > 1) {
                 // w^q from at most three table reads (w, w^2, w^4) and at most
                 // two products, so every twiddle stays within a few ulp
@@ -174,14 +187,15 @@ __device__ void fft_stage_generic(float2* buf, int ldm, int nlines, int M, int N
     const int total = nlines * M;
     const int nt = blockDim.x;
     const int tstep = M / (Ns * R);
+    const FDiv dM(M), dnbf(nbf), dNs(Ns);
     float2 out[EPT];
 #pragma unroll
     for (int c = 0; c < EPT; ++c) {
         int t = threadIdx.x + c * nt;
         if (t < total) {
-            int b = t / M, o = t - b * M;
-            int p = o / nbf, i = o - p * nbf;
-            int k = i % Ns;
+            int b = dM.quo(t), o = t - b * M;
+            int p = dnbf.quo(o), i = o - p * nbf;
+            int k = dNs.rem(i);
             int e = tstep * (k + p * Ns);  // exponent of W_M per input index q
             const int src = b * ldm + i;
             float2 acc = make_float2(0.f, 0.f);
@@ -207,9 +221,9 @@ __device__ void fft_stage_generic(float2* buf, int ldm, int nlines, int M, int N
     for (int c = 0; c < EPT; ++c) {
         int t = threadIdx.x + c * nt;
         if (t < total) {
-            int b = t / M, o = t - b * M;
-            int p = o / nbf, i = o - p * nbf;
-            int k = i % Ns;
+            int b = dM.quo(t), o = t - b * M;
+            int p = dnbf.quo(o), i = o - p * nbf;
+            int k = dNs.rem(i);
             buf[lidx(b * ldm + (i - k) * R + k + p * Ns)] = out[c];
         }
     }
@@ -243,9 +257,10 @@ __device__ bool fft_stage_oddsym(float2* buf, int ldm, int nlines, int M, int Ns
     for (int c = 0; c < MAXW; ++c) {
         int t = threadIdx.x + c * nt;
         if (t < total) {
-            int g = t % G, bi = t / G;
-            int b = bi / nbf, i = bi - b * nbf;
-            int k = i % Ns;
+            const FDiv dG(G), dnbf(nbf), dNs(Ns);
+            int bi = dG.quo(t), g = t - bi * G;
+            int b = dnbf.quo(bi), i = bi - b * nbf;
+            int k = dNs.rem(i);
             const int src = b * ldm + i;
             const int p0 = 1 + g * PP;
             const float2 x0 = buf[lidx(src)];
@@ -299,9 +314,10 @@ __device__ bool fft_stage_oddsym(float2* buf, int ldm, int nlines, int M, int Ns
     for (int c = 0; c < MAXW; ++c) {
         int t = threadIdx.x + c * nt;
         if (t < total) {
-            int g = t % G, bi = t / G;
-            int b = bi / nbf, i = bi - b * nbf;
-            int k = i % Ns;
+            const FDiv dG(G), dnbf(nbf), dNs(Ns);
+            int bi = dG.quo(t), g = t - bi * G;
+            int b = dnbf.quo(bi), i = bi - b * nbf;
+            int k = dNs.rem(i);
             const int dst = b * ldm + (i - k) * R + k;
             const int p0 = 1 + g * PP;
             if (g == 0) buf[lidx(dst)] = y0[c];
@@ -344,10 +360,11 @@ __device__ bool fft_stage_oddsym_t(float2* buf, int ldm, int nlines, int M, int 
     float2 ya[PP], yb[PP], y0 = make_float2(0.f, 0.f);
     int g = 0, dst = 0;
     if (t < total) {
-        g = t % G;
-        const int bi = t / G;
-        const int b = bi / nbf, i = bi - b * nbf;
-        const int k = i % Ns;
+        const FDiv dG(G), dnbf(nbf), dNs(Ns);
+        const int bi = dG.quo(t);
+        g = t - bi * G;
+        const int b = dnbf.quo(bi), i = bi - b * nbf;
+        const int k = dNs.rem(i);
         const int src = b * ldm + i;
         dst = b * ldm + (i - k) * R + k;
         const float2 x0 = buf[lidx(src)];
@@ -441,8 +458,9 @@ __device__ void fft_lds(float2* buf, int ldm, int nlines, const FftPlan& plan, c
 template <class Idx>
 __device__ inline void r2c_post_at(float2* buf, int nlines, int h, const float2* __restrict__ twN, Idx idx) {
     const int npairs = h / 2 + 1;
+    const FDiv dnp(npairs);
     for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
-        int b = t / npairs, k = t - b * npairs;
+        int b = dnp.quo(t), k = t - b * npairs;
         if (k == 0) {
             float2 z = buf[idx(b, 0)];
             buf[idx(b, 0)] = make_float2(z.x + z.y, z.x - z.y);
@@ -466,8 +484,9 @@ __device__ inline void r2c_post_at(float2* buf, int nlines, int h, const float2*
 template <class Idx>
 __device__ inline void c2r_pre_at(float2* buf, int nlines, int h, const float2* __restrict__ twN, Idx idx) {
     const int npairs = h / 2 + 1;
+    const FDiv dnp(npairs);
     for (int t = threadIdx.x; t < nlines * npairs; t += blockDim.x) {
-        int b = t / npairs, k = t - b * npairs;
+        int b = dnp.quo(t), k = t - b * npairs;
         if (k == 0) {
             float2 p = buf[idx(b, 0)];  // X0 + i Xh
             buf[idx(b, 0)] = make_float2(p.x + p.y, p.x - p.y);

@@ -351,12 +351,13 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
     const int z0 = blockIdx.z * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     const int2* rng = range ? range + (size_t)span * Nr + row0 : nullptr;
+    const FDiv dh(h), dR(R);
     for (int z = z0; z < z1; ++z) {
         const float* Fz = F + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
         stage_in(
             R * h,
             [&](int p) {
-                int r = p / h, e = p - r * h;
+                int r = dh.quo(p), e = p - r * h;
                 int2 lr = rng ? rng[r] : make_int2(0, Nt);
                 int th = 2 * e;
                 float2 v = make_float2(0.f, 0.f);
@@ -368,7 +369,7 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
                 return v;
             },
             [&](int p, float2 v) {
-                int r = p / h, e = p - r * h;
+                int r = dh.quo(p), e = p - r * h;
                 smc[lidx(r * ldm + e)] = v;
             });
         __syncthreads();
@@ -376,7 +377,7 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
         r2c_post(smc, ldm, R, h, rp.twN);
         float2* out = rp.S + ((size_t)span * rp.nz + z) * h * (size_t)Nr + row0;
         for (int q = threadIdx.x; q < R * h; q += blockDim.x) {
-            int f = q / R, r = q - f * R;
+            int f = dR.quo(q), r = q - f * R;
             out[(size_t)f * Nr + r] = smc[lidx(r * ldm + f)];
         }
         __syncthreads();
@@ -723,16 +724,17 @@ __global__ void k_rows_inv(RowPass rp, float* __restrict__ G, const int2* __rest
     const int row0 = blockIdx.x * R, span = blockIdx.y;
     const int z0 = blockIdx.z * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
+    const FDiv dh(h), dR(R);
     for (int z = z0; z < z1; ++z) {
         const float2* in = rp.S + ((size_t)span * rp.nz + z) * h * (size_t)Nr + row0;
         stage_in(
             R * h,
             [&](int q) {
-                int f = q / R, r = q - f * R;
+                int f = dR.quo(q), r = q - f * R;
                 return in[(size_t)f * Nr + r];
             },
             [&](int q, float2 v) {
-                int f = q / R, r = q - f * R;
+                int f = dR.quo(q), r = q - f * R;
                 smc[lidx(r * ldm + f)] = v;
             });
         __syncthreads();
@@ -740,7 +742,7 @@ __global__ void k_rows_inv(RowPass rp, float* __restrict__ G, const int2* __rest
         fft_lds<EPT>(smc, ldm, R, rp.planh, rp.twh, +1);
         float* out = G + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
         for (int p = threadIdx.x; p < R * h; p += blockDim.x) {
-            int r = p / h, e = p - r * h;
+            int r = dh.quo(p), e = p - r * h;
             int2 wr = wrange ? wrange[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
             if (2 * e + 1 >= wr.x && 2 * e < wr.y)
                 *reinterpret_cast<float2*>(out + (size_t)r * Nt + 2 * e) = smc[lidx(r * ldm + e)];
@@ -770,7 +772,7 @@ __global__ void k_cols(RowPass rp, FftPlan planM, const float2* __restrict__ twM
         fft_lds<EPT>(smc, M, 1, planM, twM, -1);
         if (f == 0) {
             for (int k = threadIdx.x; k <= M / 2; k += blockDim.x) {
-                int km = (M - k) % M;
+                int km = k == 0 ? 0 : M - k;
                 float2 A = smc[lidx(k)], Bc = cconj(smc[lidx(km)]);
                 float2 D = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
                 float2 E = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
