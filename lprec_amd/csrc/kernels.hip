@@ -297,11 +297,24 @@ __global__ void k_col_iir(const float* __restrict__ in, float* __restrict__ outp
     constexpr int NV = T + 2 * K;
     const float* col = in + (size_t)z * L * W + x;
     float* ocol = outp + (size_t)z * L * W + x;
+    // Interior chunks load without per-row conditions, so that all loads are
+    // in flight together (a guarded load compiles to a branch and a wait of
+    // its own); r0 - K is even, so with stuff 2 the data rows are the even n.
     float v[NV];
+    const bool interior = r0 >= K && r0 + T + K <= L;
+    const float* c0 = col + (size_t)(r0 - K) * W;
+    if (interior && stuff == 1) {
 #pragma unroll
-    for (int n = 0; n < NV; ++n) {
-        int row = r0 - K + n;   // uniform over the block
-        v[n] = (row >= 0 && row < L && row % stuff == 0) ? col[(size_t)row * W] : 0.f;
+        for (int n = 0; n < NV; ++n) v[n] = c0[(size_t)n * W];
+    } else if (interior && stuff == 2) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) v[n] = (n & 1) ? 0.f : c0[(size_t)n * W];
+    } else {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            int row = r0 - K + n;   // uniform over the block
+            v[n] = (row >= 0 && row < L && row % stuff == 0) ? col[(size_t)row * W] : 0.f;
+        }
     }
     const float zp = kPole;
     if (r0 == 0) {
@@ -1205,18 +1218,28 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
     const int2 lr = range ? range[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
     float2* rowbuf = smc + r * RS;
     for (int z = z0; z < z1; ++z) {
+        // all 16 loads in flight at once: unconditional, at a row base plus a
+        // 32-bit lane offset clamped into the sampled range (the clamped pairs
+        // lie in lines that are loaded anyway); pairs outside it are zeroed
+        // The index math is made opaque per slice so that the compiler does not
+        // keep 16 hoisted addresses and masks live across the loop: at the
+        // 128-VGPR budget of a 1024-lane block it would then issue the loads
+        // one at a time.
         const float* Fr = F + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt);
+        int l2 = 2 * li, rx = lr.x, ry = lr.y;
+        asm volatile("" : "+v"(l2), "+v"(rx), "+v"(ry));
+        const int tlo = min(max(rx & ~1, 0), Nt - 2), thi = max(tlo, min((ry - 1) & ~1, Nt - 2));
         float2 v[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            int th = 2 * (li + q * T16);
-            float2 x = make_float2(0.f, 0.f);
-            if (th + 1 >= lr.x && th < lr.y) {
-                x = *reinterpret_cast<const float2*>(Fr + th);
-                if (th < lr.x) x.x = 0.f;
-                if (th + 1 >= lr.y) x.y = 0.f;
-            }
-            v[q] = x;
+            const int th = l2 + 2 * q * T16;
+            v[q] = ld_at(reinterpret_cast<const float2*>(Fr), (unsigned)min(max(th, tlo), thi) * 4u);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int th = l2 + 2 * q * T16;
+            v[q].x = (th >= rx && th < ry) ? v[q].x : 0.f;
+            v[q].y = (th + 1 >= rx && th + 1 < ry) ? v[q].y : 0.f;
         }
         dft16(v, -1);
 #pragma unroll
@@ -1232,7 +1255,7 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
         row_sync<WAVE>();
         r2c_post_row<H, T16>(rowbuf, rp.twN, li);
-        __syncthreads();   // the transposed store reads every row
+        lds_barrier();   // the transposed store reads every row
         {
             // thread (rr, f0) = (tid mod R, tid / R) stores frequencies f0 + i T16
             // (i < 16) of row rr: LDS slots at fixed steps, global addresses a
@@ -1247,7 +1270,8 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
 #pragma unroll
             for (int i = 0; i < 16; ++i) st_at(out + (size_t)(i * T16) * Nr, lane, w[i]);
         }
-        __syncthreads();
+        // LDS reuse only: the stores of S drain while the next slice runs
+        lds_barrier();
     }
 }
 
@@ -1284,7 +1308,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
 #pragma unroll
             for (int i = 0; i < 16; ++i) dst[i * (T16 + T16 / 16)] = w[i];
         }
-        __syncthreads();   // every row was written by the whole block
+        lds_barrier();   // every row was written by the whole block
         c2r_pre_row<H, T16>(rowbuf, rp.twN, li);
         row_sync<WAVE>();
         float2 v[16];
@@ -1307,7 +1331,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
         row_sync<WAVE>();
         float2* out = reinterpret_cast<float2*>(G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt));
         for (int e = e0 + li; e < e1; e += T16) out[e] = rowbuf[lidx(e)];
-        __syncthreads();   // the next slice's staging overwrites every row
+        lds_barrier();   // the next slice's staging overwrites every row (stores drain meanwhile)
     }
 }
 
