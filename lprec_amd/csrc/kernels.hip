@@ -156,9 +156,17 @@ __global__ void k_image_rows(const float* __restrict__ f, float* __restrict__ fc
     const int yi = blockIdx.x, z = blockIdx.y;
     const bool rowin = yi >= o && yi < o + N0;
     const float* src = f + ((size_t)z * N0 + (rowin ? yi - o : 0)) * N0;
-    for (int x = threadIdx.x; x < N; x += blockDim.x) {
-        int xs = x - o;
-        line[x] = (rowin && xs >= 0 && xs < N0) ? src[xs] : 0.f;
+    // 8 unconditional loads in flight per lane (clamped into the row, zeroed
+    // outside it), then the LDS stores
+    for (int x0 = threadIdx.x; x0 < N; x0 += 8 * blockDim.x) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[min(max(x0 + u * (int)blockDim.x - o, 0), N0 - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int x = x0 + u * blockDim.x, xs = x - o;
+            if (x < N) line[x] = (rowin && xs >= 0 && xs < N0) ? v[u] : 0.f;
+        }
     }
     __syncthreads();
     if (cubic) iir_line_lds(line, tmp, N, threadIdx.x, blockDim.x);
@@ -215,9 +223,14 @@ __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc,
     // N + N floats after the FFT buffer (which is only allocated with a filter)
     float* line = FILT ? reinterpret_cast<float*>(smc + lds_slots(sp.Lh)) : reinterpret_cast<float*>(smc);
     float* tmp = line + N;
-    for (int x = threadIdx.x; x < N; x += blockDim.x) {
-        int xs = min(max(x - sp.lb, 0), sp.N0 - 1);
-        line[x] = src[xs];
+    // 8 loads in flight per lane, then the LDS stores
+    for (int x0 = threadIdx.x; x0 < N; x0 += 8 * blockDim.x) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[min(max(x0 + u * (int)blockDim.x - sp.lb, 0), sp.N0 - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (x0 + u * (int)blockDim.x < N) line[x0 + u * blockDim.x] = v[u];
     }
     if (FILT) {
         __syncthreads();
