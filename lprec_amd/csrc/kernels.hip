@@ -80,8 +80,21 @@ __device__ __forceinline__ float sample(const float* __restrict__ T, int W, int 
     return (1.f - ay) * ((1.f - ax) * r0[c0] + ax * r0[c1]) + ay * ((1.f - ax) * r1[c0] + ax * r1[c1]);
 }
 
+// Loads and stores at a block-uniform base plus a zero-extended 32-bit byte
+// offset per lane (the base stays in scalar registers, no 64-bit lane math).
+template <class T>
+__device__ __forceinline__ T ld_at(const T* base, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T* base, unsigned off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
 // Global -> LDS staging with 8 independent loads in flight per thread.
-// load(i) returns element i (0 <= i < n), store(i, v) writes it to LDS.
+// load(i) returns element i (0 <= i < n), store(i, v) writes it to LDS.  The
+// loads are unconditional (lanes past n reload element n - 1): a guarded load
+// compiles to a branch and a wait of its own.
 template <class Load, class Store>
 __device__ __forceinline__ void stage_in(int n, Load load, Store store) {
     constexpr int U = 8;
@@ -89,10 +102,7 @@ __device__ __forceinline__ void stage_in(int n, Load load, Store store) {
     for (int i0 = threadIdx.x; i0 < n; i0 += U * nt) {
         float2 v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            int i = i0 + u * nt;
-            if (i < n) v[u] = load(i);
-        }
+        for (int u = 0; u < U; ++u) v[u] = load(min(i0 + u * nt, n - 1));
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int i = i0 + u * nt;
@@ -376,22 +386,26 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
     const int row0 = blockIdx.x * R, span = blockIdx.y;
     const int z0 = blockIdx.z * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
-    const int2* rng = range ? range + (size_t)span * Nr + row0 : nullptr;
+    // sampled theta range of each of the block's rows (the host keeps R <= 8
+    // for these kernels)
+    __shared__ int2 rng_s[8];
+    if (threadIdx.x < R) rng_s[threadIdx.x] = range ? range[(size_t)span * Nr + row0 + threadIdx.x] : make_int2(0, Nt);
+    __syncthreads();
     const FDiv dh(h), dR(R);
     for (int z = z0; z < z1; ++z) {
         const float* Fz = F + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
         stage_in(
             R * h,
             [&](int p) {
-                int r = dh.quo(p), e = p - r * h;
-                int2 lr = rng ? rng[r] : make_int2(0, Nt);
-                int th = 2 * e;
-                float2 v = make_float2(0.f, 0.f);
-                if (th + 1 >= lr.x && th < lr.y) {
-                    v = *reinterpret_cast<const float2*>(Fz + (size_t)r * Nt + th);
-                    if (th < lr.x) v.x = 0.f;
-                    if (th + 1 >= lr.y) v.y = 0.f;
-                }
+                // unconditional load at an index clamped into the row's range
+                // (its lines are loaded anyway); pairs outside it are zeroed
+                const int r = dh.quo(p), th = 2 * (p - r * h);
+                const int2 lr = rng_s[r];
+                const int tlo = min(max(lr.x & ~1, 0), Nt - 2), thi = max(tlo, min((lr.y - 1) & ~1, Nt - 2));
+                float2 v = ld_at(reinterpret_cast<const float2*>(Fz),
+                                 (unsigned)(r * Nt + min(max(th, tlo), thi)) * 4u);
+                v.x = (th >= lr.x && th < lr.y) ? v.x : 0.f;
+                v.y = (th + 1 >= lr.x && th + 1 < lr.y) ? v.y : 0.f;
                 return v;
             },
             [&](int p, float2 v) {
@@ -420,16 +434,6 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // KIND only names the instance (0 image->LP, 1 sinogram->LP, 2 LP->polar,
 // 3 LP->Cartesian) so that profiles separate the four gathers.
 
-// Loads and stores at a block-uniform base plus a zero-extended 32-bit byte
-// offset per lane (the base stays in scalar registers, no 64-bit lane math).
-template <class T>
-__device__ __forceinline__ T ld_at(const T* base, unsigned off) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
-}
-template <class T>
-__device__ __forceinline__ void st_at(T* base, unsigned off, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
-}
 
 // Row and column of point p of a tile with nc columns, from a float
 // reciprocal (exact for p, nc < 2^20) instead of an integer division.
