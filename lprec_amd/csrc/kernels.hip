@@ -435,6 +435,20 @@ __global__ void k_rows_fwd(RowPass rp, const float* __restrict__ F, const int2* 
 // 3 LP->Cartesian) so that profiles separate the four gathers.
 
 
+// Lane (lr, lc) of a gather tile with nc = 2^lnc columns: the lane owns the
+// points of rows lr + e * rstep (rstep = kGatherThreads / nc), column lc.
+// The lanes of a wave form a block of up to 16 columns by 64 / 16 rows (one
+// row of 64 columns would spread its 16-tap reads along an arc of the source
+// box and over few rows of LDS banks): bits 0-3 of the thread index give the
+// column within a 16-wide band, the next log2(rstep) bits the row, the rest
+// the band.
+__device__ __forceinline__ void gather_lane(int lnc, int& lr, int& lc) {
+    const int t = threadIdx.x;
+    const int lbw = min(lnc, 4), lrs = __builtin_ctz((unsigned)kGatherThreads) - lnc;
+    lc = (t & ((1 << lbw) - 1)) | ((t >> (lbw + lrs)) << lbw);
+    lr = (t >> lbw) & ((1 << lrs) - 1);
+}
+
 // Row and column of point p of a tile with nc columns, from a float
 // reciprocal (exact for p, nc < 2^20) instead of an integer division.
 __device__ __forceinline__ void tile_rc(int p, int nc, float inv_nc, int& r, int& c) {
@@ -576,7 +590,8 @@ __global__ void __launch_bounds__(kGatherThreads, 5) k_gather1(Gather g, const G
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int zg = lb % nzg, tile = lb / nzg;
     const GTile t = tiles[tile];
-    const int lr = threadIdx.x >> t.lnc, lc = threadIdx.x & (t.nc - 1);
+    int lr, lc;
+    gather_lane(t.lnc, lr, lc);
     const int rstep = kGatherThreads >> t.lnc;
     const unsigned o_lane = (unsigned)(lr * g.o_row + lc) * 4u;
     const unsigned c_lane = (unsigned)(lr * g.c_row + lc) * 8u;
@@ -680,7 +695,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile
     // the tile has nc = 2^lnc columns: lane (lr, lc) holds the points of rows
     // lr + e * rstep, column lc, so every per-point address is a uniform base
     // (row e * rstep) plus a fixed lane offset
-    const int lr = threadIdx.x >> t0.lnc, lc = threadIdx.x & (t0.nc - 1);
+    int lr, lc;
+    gather_lane(t0.lnc, lr, lc);
     const int rstep = kGatherThreads >> t0.lnc;
     const unsigned o_lane = (unsigned)(lr * g.o_row + lc) * 4u;
     const unsigned c_lane = (unsigned)(lr * g.c_row + lc) * 8u;
