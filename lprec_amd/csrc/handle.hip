@@ -478,7 +478,8 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
                 if (spanrow[(size_t)a * os] != k) return h->fail(LP_ERR_UNSUPPORTED, "span rows are not contiguous");
             auto coord = [&](int, int a, int x) { return p2lp[(size_t)a * os * N + x + lb]; };
             auto keep = [](int, int, int, int) { return true; };
-            auto t = tile_grid(coord, {k}, ab, ae, N0, 8, 256, h->box_cap, keep, h->boxP);
+            // 32 angles x 64 cells: compact source boxes and blocked lanes
+            auto t = tile_grid(coord, {k}, ab, ae, N0, 32, 64, h->box_cap, keep, h->boxP);
             tP.insert(tP.end(), t.begin(), t.end());
         }
     } catch (const std::exception& e) {
