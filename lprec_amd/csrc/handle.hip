@@ -464,7 +464,7 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
                 }
                 return false;
             };
-            auto t = tile_grid(coord, {k}, 0, Nr, Nt, 32, 64, h->box_cap, keep, h->boxF);
+            auto t = tile_grid(coord, {k}, 0, Nr, Nt, 64, 32, h->box_cap, keep, h->boxF);
             tF.insert(tF.end(), t.begin(), t.end());
         }
         // LP -> polar gather over the kept angles (user rows), per span
@@ -478,8 +478,7 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
                 if (spanrow[(size_t)a * os] != k) return h->fail(LP_ERR_UNSUPPORTED, "span rows are not contiguous");
             auto coord = [&](int, int a, int x) { return p2lp[(size_t)a * os * N + x + lb]; };
             auto keep = [](int, int, int, int) { return true; };
-            // 32 angles x 64 cells: compact source boxes and blocked lanes
-            auto t = tile_grid(coord, {k}, ab, ae, N0, 32, 64, h->box_cap, keep, h->boxP);
+            auto t = tile_grid(coord, {k}, ab, ae, N0, 64, 32, h->box_cap, keep, h->boxP);
             tP.insert(tP.end(), t.begin(), t.end());
         }
     } catch (const std::exception& e) {
@@ -585,14 +584,14 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
                 }
                 return false;
             };
-            auto t = tile_grid(coord, {k}, 0, Nr, Nt, 32, 64, h->box_cap, keep, h->boxA);
+            auto t = tile_grid(coord, {k}, 0, Nr, Nt, 64, 32, h->box_cap, keep, h->boxA);
             tA.insert(tA.end(), t.begin(), t.end());
         }
         // LP -> Cartesian: user image tiles, the three spans of a tile consecutive
         const int o = N / 2 - h->N0 / 2;
         auto coordC = [&](int k, int y, int x) { return cmapC[(size_t)k * NN + (size_t)(y + o) * N + x + o]; };
         auto keepC = [](int, int, int, int) { return true; };
-        tC = tile_grid(coordC, {0, 1, 2}, 0, h->N0, h->N0, 32, 64, h->box_cap, keepC, h->boxC);
+        tC = tile_grid(coordC, {0, 1, 2}, 0, h->N0, h->N0, 64, 32, h->box_cap, keepC, h->boxC);
     } catch (const std::exception& e) {
         return h->fail(LP_ERR_UNSUPPORTED, e.what());
     }
