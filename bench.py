@@ -127,20 +127,20 @@ def pmc_summary(root, copy_mib):
 # recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
 MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536918691, 536870912),
-    "fwd.col_iir": (1026175509, 536870912),
-    "fwd.gather_lp": (4402107407, 1134007968),
-    "fwd.rows_fwd": (1094833211, 3221225472),
-    "fwd.cols": (3268676619, 3223289984),
-    "fwd.rows_inv": (3222256397, 1623806208),
-    "fwd.polar_out": (4603617899, 393216000),
-    "adj.sino_rows": (393254998, 786432000),
-    "adj.col_iir": (1522345860, 786432000),
-    "adj.gather_lp": (3762465105, 1610872496),
-    "adj.rows_fwd": (1617530984, 3221225472),
-    "adj.cols": (3270321531, 3223289856),
-    "adj.rows_inv": (3222230286, 1084827984),
-    "adj.cart_out": (4601513344, 536870912),
+    "fwd.prep_rows": (536929431, 536870912),
+    "fwd.col_iir": (555029415, 536870912),
+    "fwd.gather_lp": (3013529844, 1120087232),
+    "fwd.rows_fwd": (1105226621, 3221225472),
+    "fwd.cols": (3268846080, 3222798336),
+    "fwd.rows_inv": (3222321221, 1623789024),
+    "fwd.polar_out": (2117340677, 397426784),
+    "adj.sino_rows": (393259661, 393216000),
+    "adj.col_iir": (397792609, 786432000),
+    "adj.gather_lp": (2109088533, 1612980176),
+    "adj.rows_fwd": (1616794443, 3221225472),
+    "adj.cols": (3269324475, 3222798336),
+    "adj.rows_inv": (3222276805, 1084019760),
+    "adj.cart_out": (3626168811, 536874496),
 }
 
 
