@@ -1250,6 +1250,19 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     const int2 lr = range ? range[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
     float2* rowbuf = smc + r * RS;
+    // Up to H = 2048 the block's twiddles (twh[0, H/2), twN[0, H/2]) fit in
+    // LDS after the rows: loaded once, they spare every slice three waits on
+    // L2 after its barriers.
+    constexpr bool TWS = MID <= 8;
+    const float2* twh = rp.twh;
+    const float2* twN = rp.twN;
+    if constexpr (TWS) {
+        float2* ts = smc + R * RS;
+        for (int i = threadIdx.x; i <= H; i += blockDim.x) ts[i] = i < H / 2 ? rp.twh[i] : rp.twN[i - H / 2];
+        __syncthreads();
+        twh = ts;
+        twN = ts + H / 2;
+    }
     for (int z = z0; z < z1; ++z) {
         // all 16 loads in flight at once: unconditional, at a row base plus a
         // 32-bit lane offset clamped into the sampled range (the clamped pairs
@@ -1278,16 +1291,16 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li * 16 + q)] = v[q];
         row_sync<WAVE>();
-        if (MID > 1) row_mid_stage<MID, WAVE>(rowbuf, -1, rp.twh, li);
+        if (MID > 1) row_mid_stage<MID, WAVE>(rowbuf, -1, twh, li);
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
-        twiddle16(v, rp.twh, li, -1);
+        twiddle16(v, twh, li, -1);
         dft16(v, -1);
         row_sync<WAVE>();
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
         row_sync<WAVE>();
-        r2c_post_row<H, T16>(rowbuf, rp.twN, li);
+        r2c_post_row<H, T16>(rowbuf, twN, li);
         lds_barrier();   // the transposed store reads every row
         {
             // thread (rr, f0) = (tid mod R, tid / R) stores frequencies f0 + i T16
@@ -1328,6 +1341,18 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
     const int2 wr = wrange ? wrange[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
     const int e0 = max(wr.x, 0) >> 1, e1 = min((wr.y + 1) >> 1, H);
     float2* rowbuf = smc + r * RS;
+    // (k_rows16_fwd stages its twiddles in LDS; the same staging measured
+    // 4 % slower here, so this pass reads them from the tables)
+    constexpr bool TWS = false;
+    const float2* twh = rp.twh;
+    const float2* twN = rp.twN;
+    if constexpr (TWS) {
+        float2* ts = smc + R * RS;
+        for (int i = threadIdx.x; i <= H; i += blockDim.x) ts[i] = i < H / 2 ? rp.twh[i] : rp.twN[i - H / 2];
+        __syncthreads();
+        twh = ts;
+        twN = ts + H / 2;
+    }
     for (int z = z0; z < z1; ++z) {
         {
             // transposed load, the mirror of k_rows16_fwd's store
@@ -1342,7 +1367,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
             for (int i = 0; i < 16; ++i) dst[i * (T16 + T16 / 16)] = w[i];
         }
         lds_barrier();   // every row was written by the whole block
-        c2r_pre_row<H, T16>(rowbuf, rp.twN, li);
+        c2r_pre_row<H, T16>(rowbuf, twN, li);
         row_sync<WAVE>();
         float2 v[16];
 #pragma unroll
@@ -1352,10 +1377,10 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
 #pragma unroll
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li * 16 + q)] = v[q];
         row_sync<WAVE>();
-        if (MID > 1) row_mid_stage<MID, WAVE>(rowbuf, +1, rp.twh, li);
+        if (MID > 1) row_mid_stage<MID, WAVE>(rowbuf, +1, twh, li);
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
-        twiddle16(v, rp.twh, li, +1);
+        twiddle16(v, twh, li, +1);
         dft16(v, +1);
         // back through the row's LDS slots, then only the columns read later
         row_sync<WAVE>();
@@ -1470,7 +1495,9 @@ static int rows16_mid(const RowPass& rp) {
     return h / 256;
 }
 static size_t rows16_lds(const RowPass& rp) {
-    return (size_t)rp.R * (rp.h + rp.h / 16 + 32 / rp.R) * sizeof(float2) + 64;
+    // the rows, then (h <= 2048) the staged twiddles twh[0, h/2) and twN[0, h/2]
+    const size_t tws = rp.h <= 2048 ? (size_t)(rp.h + 1) * sizeof(float2) : 0;
+    return (size_t)rp.R * (rp.h + rp.h / 16 + 32 / rp.R) * sizeof(float2) + tws + 64;
 }
 
 #define LP_DISPATCH_MID(mid, KER, grid, block, lds, stream, ...)                                                  \
