@@ -1805,36 +1805,49 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
     const GTile t0 = tb[0];
     const int npts = t0.nr * t0.nc;
     const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    // Loads are unconditional at clamped point indices (a guarded load
+    // compiles to a branch and a wait of its own), all PPT in flight at once.
+    const float inv_nc0 = 1.0f / (float)t0.nc;
     for (int z = z0; z < z1; ++z) {
         float val[PPT];
         const float* V = g.val + (size_t)z * g.v_z + (size_t)t0.span * g.v_span;
 #pragma unroll
         for (int e = 0; e < PPT; ++e) {
-            int p = threadIdx.x + e * nt;
-            val[e] = 0.f;
-            if (p < npts) {
+            int r, c;
+            tile_rc(min(threadIdx.x + e * nt, npts - 1), t0.nc, inv_nc0, r, c);
+            val[e] = V[(size_t)(t0.r0 + r) * g.v_row + t0.c0 + c];
+        }
+        if (g.scale) {   // uniform
+#pragma unroll
+            for (int e = 0; e < PPT; ++e) {
                 int r, c;
-                tile_rc(p, t0.nc, 1.0f / (float)t0.nc, r, c);
-                float v = V[(size_t)(t0.r0 + r) * g.v_row + t0.c0 + c];
-                if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
-                val[e] = v;
+                tile_rc(min(threadIdx.x + e * nt, npts - 1), t0.nc, inv_nc0, r, c);
+                val[e] *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + c];
             }
         }
+#pragma unroll
+        for (int e = 0; e < PPT; ++e)
+            if (threadIdx.x + e * nt >= npts) val[e] = 0.f;
         for (int gi = 0; gi < g.group; ++gi) {
             const GTile t = tb[gi];
             if (t.bw <= 0) continue;   // uniform over the block
             const int bs = t.bw | 1;
             const int nbox = bs * t.bh;
+            const float2* cbase = g.coord + (size_t)t.span * g.c_span;
+            float2 cc[PPT];
+#pragma unroll
+            for (int e = 0; e < PPT; ++e) {
+                int r, cidx;
+                tile_rc(min(threadIdx.x + e * nt, npts - 1), t.nc, 1.0f / (float)t.nc, r, cidx);
+                cc[e] = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
+            }
             for (int i = threadIdx.x; i < nbox; i += nt) box[i] = 0.f;
             __syncthreads();
-            const float2* cbase = g.coord + (size_t)t.span * g.c_span;
-#pragma unroll 2
+#pragma unroll
             for (int e = 0; e < PPT; ++e) {
                 int p = threadIdx.x + e * nt;
                 if (p < npts && val[e] != 0.f) {
-                    int r, cidx;
-                    tile_rc(p, t.nc, 1.0f / (float)t.nc, r, cidx);
-                    float2 c = cbase[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
+                    const float2 c = cc[e];
                     if (c.x == c.x) {
                         float fx = floorf(c.x), fy = floorf(c.y);
                         float ax = c.x - fx, ay = c.y - fy;
