@@ -11,6 +11,7 @@
 // gathers (main_kernels.cuh:30-84) become exact 16-tap B-spline sums from HBM
 // with periodic addressing.
 #include <algorithm>
+#include <numeric>
 
 #include "kernels.hpp"
 
@@ -1800,7 +1801,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
     const int nt = blockDim.x;
     const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
     const int lbk = xcd_block(blockIdx.x, gridDim.x);
-    const int zg = lbk % nzg, tile = lbk / nzg;
+    const int ng = gridDim.x / nzg;
+    const int zg = lbk % nzg, tile = (int)(((long long)(lbk / nzg) * g.tstride) % ng);
     const GTile* tb = tiles + (size_t)tile * g.group;
     const GTile t0 = tb[0];
     const int npts = t0.nr * t0.nc;
@@ -1931,11 +1933,18 @@ void launch_zero(float* p, size_t n, hipStream_t s) {
     if (n % 4) hipLaunchKernelGGL(k_zero_tail, dim3(1), dim3(4), 0, s, p, n4 * 4, n);
 }
 
-void launch_scatter(const Scatter& g, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
+void launch_scatter(const Scatter& g0, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
     if (ntiles <= 0) return;
-    dim3 grid((ntiles / g.group) * ((g.nz + g.zgroup - 1) / g.zgroup));
-    // whole 64-float chunks for stage_box
-    size_t lds = (size_t)((g.box_floats + 63) & ~63) * sizeof(float);
+    Scatter g = g0;
+    // tile order: a stride coprime with the number of tile groups, about 0.62
+    // of it, so that blocks running together come from far apart in the tile
+    // list (neighbouring tiles flush into the same destination pixels)
+    const int ng = ntiles / g.group;
+    int st = std::max(1, (int)(0.618 * ng));
+    while (std::gcd(st, ng) != 1) ++st;
+    g.tstride = st;
+    dim3 grid(ng * ((g.nz + g.zgroup - 1) / g.zgroup));
+    size_t lds = (size_t)g.box_floats * sizeof(float);
     switch (kind) {
         case 0: hipLaunchKernelGGL(k_scatter<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
         case 1: hipLaunchKernelGGL(k_scatter<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;

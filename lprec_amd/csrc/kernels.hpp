@@ -120,6 +120,7 @@ struct Scatter {
     long long v_z, v_span;
     int v_row;
     int nz, zgroup, group, cubic, box_floats;
+    int tstride;   // set by launch_scatter: block order over the tile groups
 };
 void launch_scatter(const Scatter& g, const GTile* tiles, int ntiles, int kind, hipStream_t s);
 // transpose of the extra samples of launch_rows_fwd: F [3][nz][Nrho][Ntheta]
