@@ -271,11 +271,26 @@ __global__ void k_sino_rows_pair(const float* __restrict__ R, float* __restrict_
             float* dz = Rc + ((size_t)z * sp.P + (size_t)(a0 + u) * sp.os + j) * N;
             for (int x = threadIdx.x; x < N; x += blockDim.x) dz[x] = 0.f;
         }
-    for (int u = 0; u < 2; ++u) {
-        const float* src = R + ((size_t)z * sp.P0 + min(a0 + u, sp.P0 - 1)) * sp.N0;
-        for (int x = threadIdx.x; x < N; x += blockDim.x) {
-            int xs = min(max(x - sp.lb, 0), sp.N0 - 1);
-            line[u * N + x] = u < nrow ? src[xs] : 0.f;
+    // both rows' loads in flight together, 4 per row per lane per round
+    {
+        const float* src0 = R + ((size_t)z * sp.P0 + a0) * sp.N0;
+        const float* src1 = R + ((size_t)z * sp.P0 + min(a0 + 1, sp.P0 - 1)) * sp.N0;
+        for (int x0 = threadIdx.x; x0 < N; x0 += 4 * blockDim.x) {
+            float v0[4], v1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int xs = min(max(x0 + u * (int)blockDim.x - sp.lb, 0), sp.N0 - 1);
+                v0[u] = src0[xs];
+                v1[u] = src1[xs];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int x = x0 + u * blockDim.x;
+                if (x < N) {
+                    line[x] = v0[u];
+                    line[N + x] = nrow > 1 ? v1[u] : 0.f;
+                }
+            }
         }
     }
     __syncthreads();
