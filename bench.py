@@ -127,20 +127,20 @@ def pmc_summary(root, copy_mib):
 # recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
 MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536929431, 536870912),
-    "fwd.col_iir": (555029415, 536870912),
-    "fwd.gather_lp": (3013529844, 1120087232),
-    "fwd.rows_fwd": (1105226621, 3221225472),
-    "fwd.cols": (3268846080, 3222798336),
-    "fwd.rows_inv": (3222321221, 1623789024),
-    "fwd.polar_out": (2117340677, 397426784),
-    "adj.sino_rows": (393259661, 393216000),
-    "adj.col_iir": (397792609, 786432000),
-    "adj.gather_lp": (2109088533, 1612980176),
-    "adj.rows_fwd": (1616794443, 3221225472),
-    "adj.cols": (3269324475, 3222798336),
-    "adj.rows_inv": (3222276805, 1084019760),
-    "adj.cart_out": (3626168811, 536874496),
+    "fwd.prep_rows": (536930299, 536870912),
+    "fwd.col_iir": (553830612, 536870912),
+    "fwd.gather_lp": (3013935596, 1120177856),
+    "fwd.rows_fwd": (1100204401, 3221225472),
+    "fwd.cols": (3269440586, 3222798400),
+    "fwd.rows_inv": (3222330269, 1623826560),
+    "fwd.polar_out": (2117210651, 397320544),
+    "adj.sino_rows": (393259722, 393216000),
+    "adj.col_iir": (397813084, 786432000),
+    "adj.gather_lp": (2108228276, 1612962544),
+    "adj.rows_fwd": (1612058405, 3221225472),
+    "adj.cols": (3267435038, 3222798336),
+    "adj.rows_inv": (3222300445, 1084048448),
+    "adj.cart_out": (3626464379, 536872832),
 }
 
 
