@@ -399,8 +399,8 @@ def main():
         print(json.dumps(pmc_summary(args.pmc_summary, args.pmc_copy_mib), indent=1))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(slices shard with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here "
+              "(slices would shard across GPUs with no exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
