@@ -222,11 +222,12 @@ template <int EPT, bool FILT>
 __global__ void k_sino_rows(const float* __restrict__ R, float* __restrict__ Rc, SinoPrep sp, int cubic) {
     extern __shared__ float2 smc[];
     const int N = sp.N;
-    const int ai = blockIdx.x, z = blockIdx.y;
+    // with the cubic prefilter the grid covers the kept angles only (the
+    // column sweep does not read the stuffed rows); without it, every row
+    const int ai = cubic ? blockIdx.x * sp.os : blockIdx.x, z = blockIdx.y;
     float* dst = Rc + ((size_t)z * sp.P + ai) * N;
-    if (ai % sp.os != 0) {   // stuffed row; the cubic column sweep does not read it
-        if (!cubic)
-            for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = 0.f;
+    if (ai % sp.os != 0) {   // stuffed row: zeros
+        for (int x = threadIdx.x; x < N; x += blockDim.x) dst[x] = 0.f;
         return;
     }
     const int a = ai / sp.os;
@@ -1467,8 +1468,8 @@ void launch_image_rows(const float* f, float* fc, int nz, int N0, int N, bool cu
 
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s) {
     if (!sp.H2) {
-        hipLaunchKernelGGL((k_sino_rows<4, false>), dim3(sp.P, nz), dim3(256), 2 * (size_t)sp.N * sizeof(float), s, R,
-                           Rc, sp, (int)cubic);
+        hipLaunchKernelGGL((k_sino_rows<4, false>), dim3(cubic ? sp.P0 : sp.P, nz), dim3(256),
+                           2 * (size_t)sp.N * sizeof(float), s, R, Rc, sp, (int)cubic);
         return;
     }
     // two kept angles per block, one complex FFT pair of length 2N
