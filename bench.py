@@ -127,20 +127,20 @@ def pmc_summary(root, copy_mib):
 # recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
 MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536930299, 536870912),
-    "fwd.col_iir": (553830612, 536870912),
-    "fwd.gather_lp": (3013935596, 1120177856),
-    "fwd.rows_fwd": (1100204401, 3221225472),
-    "fwd.cols": (3269440586, 3222798400),
-    "fwd.rows_inv": (3222330269, 1623826560),
-    "fwd.polar_out": (2117210651, 397320544),
-    "adj.sino_rows": (393259722, 393216000),
-    "adj.col_iir": (397813084, 786432000),
-    "adj.gather_lp": (2108228276, 1612962544),
-    "adj.rows_fwd": (1612058405, 3221225472),
-    "adj.cols": (3267435038, 3222798336),
-    "adj.rows_inv": (3222300445, 1084048448),
-    "adj.cart_out": (3626464379, 536872832),
+    "fwd.prep_rows": (536929391, 536870912),
+    "fwd.col_iir": (553899853, 536870912),
+    "fwd.gather_lp": (3010644222, 1120003392),
+    "fwd.rows_fwd": (1100117859, 3221225472),
+    "fwd.cols": (3268811608, 3222798336),
+    "fwd.rows_inv": (3222319765, 1623866688),
+    "fwd.polar_out": (2115516657, 397416960),
+    "adj.sino_rows": (393266509, 393216000),
+    "adj.col_iir": (397820258, 786432000),
+    "adj.gather_lp": (2107705788, 1612938352),
+    "adj.rows_fwd": (1612036738, 3221225472),
+    "adj.cols": (3268576539, 3222798336),
+    "adj.rows_inv": (3222201686, 1084026720),
+    "adj.cart_out": (3621418027, 536874432),
 }
 
 
