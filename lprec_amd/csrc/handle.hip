@@ -59,7 +59,7 @@ struct lp_handle {
     int N = 0, N0 = 0, Nt = 0, Nr = 0, P = 0, P0 = 0, Ns = 0, cor = 0, os = 1, h = 0;
     bool cubic = true;
     bool has_fwd = false, has_adj = false, has_filter = false;
-    int R = 16, zgroup = 8, colchunk = 32, gzgroup = 8, fft16 = 1;
+    int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
