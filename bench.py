@@ -127,20 +127,20 @@ def pmc_summary(root, copy_mib):
 # recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
 MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536929391, 536870912),
-    "fwd.col_iir": (553899853, 536870912),
-    "fwd.gather_lp": (3010644222, 1120003392),
-    "fwd.rows_fwd": (1100117859, 3221225472),
-    "fwd.cols": (3268811608, 3222798336),
-    "fwd.rows_inv": (3222319765, 1623866688),
-    "fwd.polar_out": (2115516657, 397416960),
-    "adj.sino_rows": (393266509, 393216000),
-    "adj.col_iir": (397820258, 786432000),
-    "adj.gather_lp": (2107705788, 1612938352),
-    "adj.rows_fwd": (1612036738, 3221225472),
-    "adj.cols": (3268576539, 3222798336),
-    "adj.rows_inv": (3222201686, 1084026720),
-    "adj.cart_out": (3621418027, 536874432),
+    "fwd.prep_rows": (536929951, 536870912),
+    "fwd.col_iir": (554485682, 536870912),
+    "fwd.gather_lp": (2841307736, 1118378848),
+    "fwd.rows_fwd": (1100124348, 3221225472),
+    "fwd.cols": (3269775394, 3222798400),
+    "fwd.rows_inv": (3222292661, 1623842176),
+    "fwd.polar_out": (2025893163, 397508672),
+    "adj.sino_rows": (393266212, 393216000),
+    "adj.col_iir": (397741240, 786432000),
+    "adj.gather_lp": (2103821933, 1612039968),
+    "adj.rows_fwd": (1612065169, 3221225472),
+    "adj.cols": (3269389798, 3222798336),
+    "adj.rows_inv": (3222230006, 1084030416),
+    "adj.cart_out": (2765951997, 536873152),
 }
 
 
