@@ -27,15 +27,24 @@ def random_ellipses(n, rng, inside=False):
 
 def rasterize(ell, N, lin=None):
     """float64 [N, N] image of the ellipse set, masked to the unit disk, at
-    pixel coordinates ``lin`` (default linspace(-1, 1, N))."""
-    lin = np.linspace(-1, 1, N) if lin is None else lin
-    X, Y = np.meshgrid(lin, lin)
+    pixel coordinates ``lin`` (default linspace(-1, 1, N), increasing).
+    Each ellipse is evaluated on the pixels of its bounding box only."""
+    lin = np.linspace(-1, 1, N) if lin is None else np.asarray(lin, np.float64)
     img = np.zeros((N, N))
     for A, a, b, x0, y0, psi in ell:
         c, s = np.cos(psi), np.sin(psi)
+        # half extents of the rotated ellipse along x and y, plus a margin
+        ex = np.sqrt((a * c) ** 2 + (b * s) ** 2) * 1.0001 + 1e-9
+        ey = np.sqrt((a * s) ** 2 + (b * c) ** 2) * 1.0001 + 1e-9
+        i0, i1 = np.searchsorted(lin, [x0 - ex, x0 + ex])
+        j0, j1 = np.searchsorted(lin, [y0 - ey, y0 + ey])
+        if i1 <= i0 or j1 <= j0:
+            continue
+        X, Y = np.meshgrid(lin[i0:i1], lin[j0:j1])
         u = (X - x0) * c + (Y - y0) * s
         v = -(X - x0) * s + (Y - y0) * c
-        img[(u / a) ** 2 + (v / b) ** 2 <= 1] += A
+        img[j0:j1, i0:i1][(u / a) ** 2 + (v / b) ** 2 <= 1] += A
+    X, Y = np.meshgrid(lin, lin)
     img[X ** 2 + Y ** 2 > 1] = 0
     return img
 

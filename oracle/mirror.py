@@ -16,9 +16,11 @@ class does, and restates:
             (what the 4 bilinear fetches at h0/h1 compute, with exact
             instead of 8-bit-fraction weights), and periodic addressing
             (cudaAddressModeWrap, lpRgpu.cu:22-23, 77-78, 114-115).
+            ``interp_taps`` is the numpy form; oracle/interp.c the same sum
+            in C (OpenMP), used when built (``python -m oracle.build``).
 
 Pins: ``prefilter_line`` against scipy.ndimage.spline_filter1d(mode='reflect');
-``interp`` against scipy.ndimage.map_coordinates(order=3, mode='grid-wrap',
+``interp`` / ``interp_taps`` against scipy.ndimage.map_coordinates(order=3, mode='grid-wrap',
 prefilter=False) and against brute-force 16-tap sums; the FFT convolution
 against a direct circular convolution on small grids; the whole pipeline
 against the reference fixtures (tests/golden/f, R) at the reference's
@@ -99,8 +101,9 @@ def bspline_weights(f):
             f ** 3 / 6.0)
 
 
-def interp(T, x, y, cubic=True):
-    """Sample the periodic grid T[..., H, W] at table coordinates (x, y).
+def interp_taps(T, x, y, cubic=True):
+    """Sample the periodic grid T[..., H, W] at table coordinates (x, y):
+    the numpy restatement of the 16-tap (4-tap linear) sum.
 
     Leading axes of T are slices; returns [..., len(x)].
     """
@@ -129,6 +132,51 @@ def interp(T, x, y, cubic=True):
             rows = ((iy + b) % H) * W
             out += wyb * ((1.0 - fx) * flat[..., rows + ix % W] + fx * flat[..., rows + (ix + 1) % W])
     return out
+
+
+_CLIB = None
+
+
+def _clib():
+    """oracle/liboracle.so (oracle/interp.c, built by oracle/build.py), or None."""
+    global _CLIB
+    if _CLIB is None:
+        import ctypes
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle.so")
+        try:
+            lib = ctypes.CDLL(path)
+            d = ctypes.POINTER(ctypes.c_double)
+            i64 = ctypes.c_int64
+            lib.oracle_interp.argtypes = [d, i64, i64, i64, d, d, i64, ctypes.c_int, d]
+            lib.oracle_interp.restype = None
+            _CLIB = lib
+        except OSError:
+            _CLIB = False
+    return _CLIB or None
+
+
+def interp(T, x, y, cubic=True):
+    """Sample the periodic grid T[..., H, W] at table coordinates (x, y).
+
+    Leading axes of T are slices; returns [..., len(x)].  Runs the C
+    restatement (oracle/interp.c) when it is built, else ``interp_taps``;
+    the two agree to 1e-12 (tests/test_oracle.py).
+    """
+    lib = _clib()
+    if lib is None:
+        return interp_taps(T, x, y, cubic)
+    import ctypes
+    H, W = T.shape[-2:]
+    lead = T.shape[:-2]
+    Tc = np.ascontiguousarray(T, dtype=np.float64).reshape(-1, H, W)
+    xc = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    yc = np.ascontiguousarray(y, dtype=np.float64).ravel()
+    n = xc.size
+    out = np.empty((Tc.shape[0], n))
+    d = ctypes.POINTER(ctypes.c_double)
+    lib.oracle_interp(Tc.ctypes.data_as(d), Tc.shape[0], H, W, xc.ctypes.data_as(d), yc.ctypes.data_as(d), n,
+                      1 if cubic else 0, out.ctypes.data_as(d))
+    return out.reshape(lead + (n,))
 
 
 def prefilter_line(c, axis):

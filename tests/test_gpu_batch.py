@@ -138,3 +138,67 @@ def test_graphed_em_matches_eager_every_replay(torch_cuda):
     print("graphed vs lpmethods.em after 6 iterations: %.2e" % e_ref, flush=True)
     assert worst < 1e-5
     assert e_ref < 1e-5
+
+
+def test_multiscan_replay_matches_mirror_per_scan(torch_cuda):
+    """The uneven multi-scan workload's geometries (N0 in {512, 1024, 2048},
+    P0 in {360, 720, 1500}, one slice each) in one captured graph: after
+    replay() every scan's sinogram and back-projection equal the oracle
+    mirror's (relative L2 <= 1e-5)."""
+    torch = torch_cuda
+    from oracle import mirror
+    from lprec_amd.batch import Scan
+    scans = [Scan(n, p, 1) for n in (512, 1024, 2048) for p in (360, 720, 1500)]
+    ms = MultiScan(scans)
+    imgs = [phantom_stack(1, s.N0, 12, seed=500 + i) for i, s in enumerate(scans)]
+    for i, im in enumerate(imgs):
+        ms.scan_views(i)[0].copy_(torch.from_numpy(im))
+    ms.capture()
+    for g in ms.groups:   # the replay must overwrite these
+        g.R.fill_(7.0)
+        g.back.fill_(-3.0)
+    torch.cuda.synchronize()
+    ms.replay()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, s in enumerate(scans):
+        g = ms.groups[ms.where[i][0]]
+        tab = mirror.Tables(g.lp.glpars, g.lp.fwdparsi, g.lp.fwdparamsf, g.lp.adjparsi, g.lp.adjparamsf)
+        _, Rg, bg = ms.scan_views(i)
+        Rm = mirror.fwd(tab, imgs[i])
+        bm = mirror.adj(tab, Rg.cpu().numpy())
+        ef, ea = rel(Rg.cpu(), Rm), rel(bg.cpu(), bm)
+        print("scan N0 %d P0 %d: fwd %.2e adj %.2e" % (s.N0, s.Nproj, ef, ea), flush=True)
+        worst = max(worst, ef, ea)
+    assert worst <= 1e-5
+
+
+def test_graphed_em_matches_fp64_oracle_em_at_2048(torch_cuda):
+    """EM at the batched EM workload's geometry (N0 = 2048, 2048 angles, 30
+    ellipses, 5e3 photons), 2 slices, one iteration per graph replay, against
+    oracle.solvers.em on the FP64 mirror after each of 3 iterations
+    (relative L2 <= 1e-5 per iterate)."""
+    torch = torch_cuda
+    from oracle import mirror, solvers
+    from lprec_amd import lpmethods
+    from lprec_amd.phantom import scan
+    N0, P0, ns = 2048, 2048, 2
+    lp = lpTransform(N0, P0, ns, "none", N0 / 2, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    _, g = scan(ns, N0, P0, 30, seed=55, photons=5e3)
+    g = np.maximum(g, 0).astype(np.float32)
+    init = np.full((ns, N0, N0), 1e-3, np.float32)
+    tab = mirror.Tables(lp.glpars, lp.fwdparsi, lp.fwdparamsf, lp.adjparsi, lp.adjparamsf)
+    _, kept = solvers.em(lambda f: mirror.fwd(tab, f), lambda R: mirror.adj(tab, R), init, g, 3,
+                         float(np.float32(1e-3)), keep=(1, 2, 3))
+    em = lpmethods.GraphedEM(lp, torch.from_numpy(init).cuda(), torch.from_numpy(g).cuda(), 1e-3, 0)
+    em.capture()
+    worst = 0.0
+    for it in (1, 2, 3):
+        em.step()
+        torch.cuda.synchronize()
+        e = rel(em.recon.cpu().numpy(), kept[it])
+        print("EM iteration %d: relative difference to the FP64 oracle %.2e" % (it, e), flush=True)
+        worst = max(worst, e)
+    assert worst <= 1e-5

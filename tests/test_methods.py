@@ -75,6 +75,61 @@ def test_em_identity_fixed_point():
     assert np.allclose(out.reshape(-1), b.reshape(-1) / (1 + 1e-5), rtol=1e-6)
 
 
+def _dense_fns(A, n):
+    ns_shape = lambda v: v.reshape(v.shape[0], -1)  # noqa: E731
+    return (lambda f: (ns_shape(f) @ A.T).reshape(f.shape[0], -1, 1),
+            lambda R: (ns_shape(R) @ A).reshape(R.shape[0], n, n))
+
+
+def test_oracle_grad_pins():
+    """oracle/solvers.py grad (FP64): identity operator with a fixed step has
+    the closed form b (1 - (1 - 2 lam)^k); the Barzilai-Borwein run reaches
+    the least-squares solution; its second step equals the BB quotient of
+    the first two iterates computed from the definition."""
+    from oracle import solvers
+    fwd, adj = _dense_fns(np.eye(9), 3)
+    b = np.random.default_rng(1).standard_normal((1, 9, 1))
+    out, _ = solvers.grad(fwd, adj, np.zeros((1, 3, 3)), b, 7, 0.1)
+    assert np.allclose(out.reshape(-1), b.reshape(-1) * (1 - 0.8 ** 7), atol=1e-12)
+    A, x, b = _problem(2)
+    fwd, adj = _dense_fns(A, 4)
+    b = b.astype(np.float64)
+    out, kept = solvers.grad(fwd, adj, np.zeros((2, 4, 4)), b, 120, -1, keep=(1, 2))
+    xls = np.stack([np.linalg.lstsq(A, b[z, :, 0], rcond=None)[0] for z in range(2)]).reshape(2, 4, 4)
+    assert np.allclose(out, xls, atol=1e-10)
+    # step 2 from the definition: s = x1 - x0, y = g1 - g0, lam = <s, y> / <y, y>
+    g = lambda v: adj(2 * (fwd(v) - b))  # noqa: E731
+    x0 = np.zeros((2, 4, 4))
+    x1 = x0 - 1e-3 * g(x0)
+    s_, y_ = x1 - x0, g(x1) - g(x0)
+    lam = np.sum(s_ * y_, axis=(1, 2)) / np.sum(y_ * y_, axis=(1, 2))
+    assert np.allclose(kept[1], x1, atol=1e-14)
+    assert np.allclose(kept[2], x1 - lam[:, None, None] * g(x1), atol=1e-12)
+
+
+def test_oracle_em_pins():
+    """oracle/solvers.py em (FP64): on a consistent positive system the
+    iterates stay positive, the Kullback-Leibler objective decreases
+    monotonically, and they approach the exact solution."""
+    from oracle import solvers
+    rng = np.random.default_rng(4)
+    n, m = 3, 20
+    A = rng.uniform(0.1, 1.0, (m, n * n))
+    x = rng.uniform(0.5, 2.0, (1, n, n))
+    fwd, adj = _dense_fns(A, n)
+    b = fwd(x)
+    kl = []
+    xk = np.ones((1, n, n))
+    for _ in range(4):
+        xk, _ = solvers.em(fwd, adj, xk, b, 500, 0.0)
+        Ax = fwd(xk)
+        kl.append(float(np.sum(b * np.log(b / Ax) - b + Ax)))
+        assert np.all(xk > 0)
+    # (the 1e-5 added to R*(1) moves the fixed point by ~1e-5)
+    assert all(k1 < k0 for k0, k1 in zip(kl, kl[1:]))
+    assert kl[-1] < 1e-5 * kl[0] and np.max(np.abs(xk - x)) < 1e-4
+
+
 @pytest.mark.parametrize("name", ["tv", "tvl1", "tve"])
 def test_chambolle_pock_without_regularisation_recovers_data(name):
     # identity operator, vanishing TV weight: the minimiser is b (b > 0 for tve)

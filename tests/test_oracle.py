@@ -41,31 +41,36 @@ def test_prefilter_then_interp_reproduces_interior_samples():
 
 
 def test_interp_matches_scipy_map_coordinates_grid_wrap():
+    # both forms of the sampler: the numpy restatement and the C one
+    # (oracle/interp.c), which the mirror runs when it is built
+    assert mirror._clib() is not None, "oracle/liboracle.so not built (python -m oracle.build)"
     rng = np.random.default_rng(2)
     T = rng.standard_normal((2, 37, 53))
-    x = rng.uniform(-6, 60, 500)
-    y = rng.uniform(-6, 44, 500)
+    x = rng.uniform(-60, 120, 2000)
+    y = rng.uniform(-60, 100, 2000)
     for cubic, order in ((True, 3), (False, 1)):
-        a = mirror.interp(T, x, y, cubic)
-        for s in range(2):
-            b = ndimage.map_coordinates(T[s], [y, x], order=order, mode="grid-wrap", prefilter=False)
-            assert np.max(np.abs(a[s] - b)) < 1e-12
+        for fn in (mirror.interp_taps, mirror.interp):
+            a = fn(T, x, y, cubic)
+            for s in range(2):
+                b = ndimage.map_coordinates(T[s], [y, x], order=order, mode="grid-wrap", prefilter=False)
+                assert np.max(np.abs(a[s] - b)) < 1e-12
 
 
 def test_interp_brute_force_16_taps():
     rng = np.random.default_rng(3)
     T = rng.standard_normal((9, 11))
-    x, y = 4.3, 2.8
-    ix, iy = 4, 2
-    fx, fy = 0.3, 0.8
 
     def b3(t):  # centred cubic B-spline
         t = abs(t)
         return (2 / 3 - t * t + t ** 3 / 2) if t < 1 else ((2 - t) ** 3 / 6 if t < 2 else 0.0)
 
-    ref = sum(b3(fx - (a - 1)) * b3(fy - (b - 1)) * T[(iy - 1 + b) % 9, (ix - 1 + a) % 11]
-              for a in range(4) for b in range(4))
-    assert abs(mirror.interp(T, [x], [y])[0] - ref) < 1e-12
+    for x, y in ((4.3, 2.8), (-0.6, 8.7), (10.9, -3.2)):
+        ix, iy = int(np.floor(x)), int(np.floor(y))
+        fx, fy = x - ix, y - iy
+        ref = sum(b3(fx - (a - 1)) * b3(fy - (b - 1)) * T[(iy - 1 + b) % 9, (ix - 1 + a) % 11]
+                  for a in range(4) for b in range(4))
+        for fn in (mirror.interp_taps, mirror.interp):
+            assert abs(fn(T, [x], [y])[0] - ref) < 1e-12
 
 
 def test_lp_convolve_is_circular_convolution():

@@ -18,6 +18,8 @@ GEOMS = [
     (250, 300, 127.3, "ramp"),      # centre offset: N = 256
     (200, 151, 100, "shepp-logan"),  # odd Nproj, os = round(1.99) = 2
     (96, 48, 48, "cosine2"),        # os = 3
+    (2048, 1500, 1024, "none"),     # the headline workload's tables (N = 2048, os = 2)
+    (2048, 1500, 1021.6, "ramp"),   # the FBP workload's centre offset: N = 2064 = 2^4 3 43
 ]
 
 
