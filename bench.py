@@ -122,30 +122,78 @@ def pmc_summary(root, copy_mib):
             "stages": {k: [int(fetch[k] * ff), int(write.get(k, 0.0) * fw)] for k in fetch}}
 
 
-# HBM traffic per launch (bytes read, bytes written) of each stage at the
-# default workload and schedule, from pmc_summary() over the rocprofv3 runs
-# recorded in profiles/lprec_fwd_adj_N2048_P1500_b32.md (copy calibration:
-# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 bytes per counted byte).
-MEASURED_TRAFFIC = {
-    "fwd.prep_rows": (536929951, 536870912),
-    "fwd.col_iir": (554485682, 536870912),
-    "fwd.gather_lp": (2841307736, 1118378848),
-    "fwd.rows_fwd": (1100124348, 3221225472),
-    "fwd.cols": (3269775394, 3222798400),
-    "fwd.rows_inv": (3222292661, 1623842176),
-    "fwd.polar_out": (2025893163, 397508672),
-    "adj.sino_rows": (393266212, 393216000),
-    "adj.col_iir": (397741240, 786432000),
-    "adj.gather_lp": (2103821933, 1612039968),
-    "adj.rows_fwd": (1612065169, 3221225472),
-    "adj.cols": (3269389798, 3222798336),
-    "adj.rows_inv": (3222230006, 1084030416),
-    "adj.cart_out": (2765951997, 536873152),
-}
+SOURCES = ("kernels.hip", "handle.hip", "fft_lds.hpp", "fft64.hpp", "kernels.hpp", "lp_internal.hpp")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fwdadj_N2048_P1500_b32.json")
 
 
-def cpu_baseline(lp, f0, budget_s):
-    """Time the FP64 oracle mirror (numpy/scipy) on one slice of the workload."""
+def sources_digest():
+    """SHA-256 over the device sources: measured traffic is valid only for the
+    kernels it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in SOURCES:
+        with open(os.path.join(ROOT, "lprec_amd", "csrc", name), "rb") as fh:
+            h.update(name.encode() + b"\0" + fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic():
+    """Per-stage HBM bytes (read, written) per launch at the default workload
+    from the committed rocprofv3 --pmc summary, or None when that summary was
+    measured on other kernel sources than the ones present."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("sources_sha256") != sources_digest():
+        return None
+    return {k: tuple(v) for k, v in rec["stages"].items()}
+
+
+def stream_ceiling(mib=4096, reps=10):
+    """GB/s of the library's streaming kernels (16-byte non-temporal loads and
+    stores) over mib MiB, far beyond the 256 MB memory-side cache: copy (read
+    + write bytes), read-only and write-only."""
+    import torch
+    from lprec_amd import _lib
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty(n, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for kind, name, nbytes in ((0, "copy", 2 * n * 4), (1, "read", n * 4), (2, "write", n * 4)):
+        _lib.stream_bench(0, b.data_ptr(), a.data_ptr(), n, kind, 0, stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            _lib.stream_bench(0, b.data_ptr(), a.data_ptr(), n, kind, 0, stream)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name + "_GBps"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
+    del a, b
+    torch.cuda.empty_cache()
+    out["buffer_MiB"] = mib
+    return out
+
+
+def _host_cpu():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(lp, f0, budget_s, R_dev=None, back_dev=None):
+    """Time the FP64 oracle mirror (numpy/scipy + the oracle's C sampler) on
+    slices of the workload; when the device outputs of slice 0 are given,
+    also report their relative L2 difference to the mirror's (fwd on the
+    same image; adj on the device's own sinogram)."""
     import numpy as np
     from oracle import mirror
     threads = len(os.sched_getaffinity(0))
@@ -155,24 +203,40 @@ def cpu_baseline(lp, f0, budget_s):
     n = 0
     while True:
         R = mirror.fwd(tab, f)
-        mirror.adj(tab, R)
+        back = mirror.adj(tab, R)
         n += 1
         el = time.perf_counter() - t0
         if el + el / n > budget_s or n >= 3:
             break
     dt = time.perf_counter() - t0
-    cpu = "unknown"
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"value": n / dt, "unit": "slices/s", "cores": threads, "kind": "port",
-            "sample": "%d slice(s) fwd+adj of the same workload through the FP64 numpy/scipy mirror "
-                      "(scipy.fft workers=%d), %.1f s, host CPU: %s" % (n, threads, dt, cpu)}
+    out = {"value": n / dt, "unit": "slices/s", "cores": threads, "kind": "port",
+           "sample": "%d slice(s) fwd+adj of the same workload through the FP64 oracle mirror "
+                     "(scipy.fft workers=%d, OpenMP sampler), %.1f s, host CPU: %s" % (n, threads, dt, _host_cpu())}
+    if R_dev is not None:
+        Rg = R_dev[:1].double().cpu().numpy()
+        bg = back_dev[:1].double().cpu().numpy()
+        bm = mirror.adj(tab, Rg)
+        out["device_vs_mirror_slice0"] = {"fwd_rel_l2": float(np.linalg.norm(Rg - R) / np.linalg.norm(R)),
+                                          "adj_rel_l2": float(np.linalg.norm(bg - bm) / np.linalg.norm(bm))}
+    return out
+
+
+def cpu_config1():
+    """The oracle mirror on all 4 slices of the small plain configuration
+    (N0 = 256, 384 angles, 12-ellipse phantoms): seconds and slices/s."""
+    import numpy as np
+    from oracle import mirror
+    from lprec_amd import lpTransform
+    from lprec_amd.phantom import phantom_stack
+    lp = lpTransform(256, 384, 4, "none", 128, "cubic")
+    lp.precompute(1)
+    tab = mirror.Tables(lp.glpars, lp.fwdparsi, lp.fwdparamsf, lp.adjparsi, lp.adjparamsf)
+    f = phantom_stack(4, 256, 12, seed=256).astype(np.float64)
+    t0 = time.perf_counter()
+    mirror.adj(tab, mirror.fwd(tab, f))
+    dt = time.perf_counter() - t0
+    return {"workload": "fwd+adj, N0=256, 384 angles, 4 slices", "seconds": round(dt, 3),
+            "value": round(4 / dt, 2), "unit": "slices/s", "cores": len(os.sched_getaffinity(0)), "kind": "port"}
 
 
 def _timed(fn, steps, warmup):
@@ -382,12 +446,16 @@ def main():
     ap.add_argument("--plain-slices", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-kernel HBM ceiling")
     ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16, boxcap)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--pmc-summary", default="", metavar="DIR",
                     help="print calibrated per-stage traffic from rocprofv3 --pmc runs under DIR and exit")
     ap.add_argument("--pmc-copy-mib", type=int, default=4096)
+    ap.add_argument("--write-traffic", action="store_true",
+                    help="with --pmc-summary: store the table, stamped with the device-source digest, as the "
+                         "traffic the default run reports")
     ap.add_argument("--workload", default="fwdadj", choices=("fwdadj", "fbp", "multiscan", "em", "autograd"),
                     help="fwdadj: the headline line; fbp: FBP at the centre-offset geometry (ramp and "
                          "parzen); multiscan: 96 uneven scans in one graph launch; em: 50 graph-replayed "
@@ -396,7 +464,13 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="multiscan: autotune every geometry first")
     args = ap.parse_args()
     if args.pmc_summary:
-        print(json.dumps(pmc_summary(args.pmc_summary, args.pmc_copy_mib), indent=1))
+        summ = pmc_summary(args.pmc_summary, args.pmc_copy_mib)
+        if args.write_traffic:
+            summ["sources_sha256"] = sources_digest()
+            summ["workload"] = "fwd+adj, N0=2048, 1500 angles, 32 slices, default schedule"
+            with open(TRAFFIC_FILE, "w") as fh:
+                json.dump(summ, fh, indent=1)
+        print(json.dumps(summ, indent=1))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path here "
@@ -503,7 +577,9 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3)
     b_alg = pipeline_bytes(g)
     default_run = (N0, P0, ns, args.tune) == (2048, 1500, 32, "")
-    traffic = sum(MEASURED_TRAFFIC[dom]) if default_run and dom in MEASURED_TRAFFIC else None
+    table = measured_traffic() if default_run else None
+    traffic = sum(table[dom]) if table and dom in table else None
+    ceiling = stream_ceiling() if not args.no_ceiling else None
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -524,7 +600,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "alg_bytes_per_launch": int(dom_bytes), "avg_launch_ms": round(dom_ms, 4),
-                     "traffic_read_write": list(MEASURED_TRAFFIC[dom]) if traffic else None},
+                     "traffic_read_write": list(table[dom]) if traffic else None,
+                     "measured_ceiling": ceiling},
         "pipeline_roofline": {"B_alg_bytes_per_slice": int(b_alg),
                               "achieved_GBps": round(b_alg * value / 1e9, 1),
                               "frac": round(b_alg * value / HBM_PEAK, 4)},
@@ -534,7 +611,8 @@ def main():
         "schedule": {k: h.get_tuning(i) for k, i in knobs.items()},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(lp, f_host, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(lp, f_host, args.cpu_budget, R, back)
+        out["cpu_config1"] = cpu_config1()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

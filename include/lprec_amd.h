@@ -84,6 +84,14 @@ const char* lp_last_error(const lp_handle* h);
 /* Replaces lpRgpu::printCurrentGPUMemory (params.cu:147-155). */
 int lp_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
 
+/* HBM streaming kernel (no reference counterpart; a measurement helper):
+ * kind 0 copies n_floats floats src -> dst (16-byte non-temporal loads and
+ * stores), 1 reads src (dst receives blocks * 256 float4 partial sums),
+ * 2 writes dst.  n_floats must be a multiple of 4; blocks <= 0 picks 2048.
+ * The benchmark reports its GB/s as the measured ceiling next to the
+ * vendor peak. */
+int lp_stream_bench(int device, float* dst, const float* src, size_t n_floats, int kind, int blocks, void* stream);
+
 /* ---- introspection and tuning (no reference counterpart) ---------------- */
 /* info[0..11] = N, N0, Ntheta, Nrho, Nproj(internal), Nproj0, os, Nslices,
  * cor(int), cubic, has_filter, device bytes held by the handle (clamped to

@@ -25,7 +25,7 @@ EXPORTS = (
     "lp_pre_create", "lp_pre_sizes", "lp_pre_copy", "lp_pre_destroy", "lp_pre_error",
     "lp_create", "lp_init_fwd", "lp_init_adj", "lp_fwd", "lp_adj", "lp_fwd_T", "lp_adj_T", "lp_destroy",
     "lp_last_error", "lp_mem_info", "lp_info", "lp_set_tuning", "lp_get_tuning",
-    "lp_profile", "lp_profile_read", "lp_debug_stop", "lp_debug_copy",
+    "lp_profile", "lp_profile_read", "lp_debug_stop", "lp_debug_copy", "lp_stream_bench",
 )
 
 
@@ -72,6 +72,7 @@ def load():
         "lp_profile_read": (i32, [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]),
         "lp_debug_stop": (i32, [vp, i32]),
         "lp_debug_copy": (i32, [vp, i32, vp, ctypes.c_size_t]),
+        "lp_stream_bench": (i32, [i32, vp, vp, ctypes.c_size_t, i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -112,6 +113,14 @@ def precompute(N0, Nproj, Nslices, cor, cubic, filter_type, want_fwd):
     finally:
         lib.lp_pre_destroy(h)
     return out
+
+
+def stream_bench(device, dst_ptr, src_ptr, n_floats, kind, blocks, stream):
+    """HBM streaming kernel (0 copy, 1 read, 2 write) on raw device pointers."""
+    rc = load().lp_stream_bench(int(device), ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr), int(n_floats),
+                                int(kind), int(blocks), ctypes.c_void_p(stream))
+    if rc != LP_OK:
+        raise LprecError(rc, "lp_stream_bench failed")
 
 
 class Handle:

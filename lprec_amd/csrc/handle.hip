@@ -652,51 +652,56 @@ static lpk::RowPass row_pass(lp_handle* h, int nz) {
     return rp;
 }
 
-int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
-    if (!h) return LP_ERR_ARG;
-    if (!h->has_fwd) return h->fail(LP_ERR_ARG, "forward tables not initialised (lp_init_fwd)");
-    if (!R || !f) return h->fail(LP_ERR_ARG, "null data pointer");
-    if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
-    DeviceGuard guard(h->device);
-    hipStream_t s = (hipStream_t)stream;
+// Device buffers of one fwd call: prefiltered image fc [ns0][N][N], the
+// row-sweep output tmp (may alias G), the log-polar grids G [3][ns0][Nr][Nt]
+// and the half spectra S [3][ns0][h][Nr].
+struct FwdScratch {
+    float* fc;
+    float* tmp;
+    float* G;
+    float2* S;
+};
+
+static int fwd_chain(lp_handle* h, float* R, const float* f, int ns0, hipStream_t s, const FwdScratch& w,
+                     bool timed) {
     (void)hipGetLastError();
     const int N = h->N;
-    hipEvent_t* ev = h->prof_begin(0);
+    hipEvent_t* ev = timed ? h->prof_begin(0) : nullptr;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     mark(0);
-    lpk::launch_image_rows(f, h->cubic ? h->d_tmp : h->d_fc, ns0, h->N0, N, h->cubic, s);
+    lpk::launch_image_rows(f, h->cubic ? w.tmp : w.fc, ns0, h->N0, N, h->cubic, s);
     mark(1);
     if (h->stop_after == 1) return check_launch(h, "lp_fwd launch");
-    if (h->cubic) lpk::launch_col_iir(h->d_tmp, h->d_fc, ns0, N, N, h->colchunk, 1, s);
+    if (h->cubic) lpk::launch_col_iir(w.tmp, w.fc, ns0, N, N, h->colchunk, 1, s);
     mark(2);
     if (h->stop_after == 2) return check_launch(h, "lp_fwd launch");
     lpk::RowPass rp = row_pass(h, ns0);
-    lpk::Src src{h->d_fc, N, N, (long long)N * N};
+    rp.S = w.S;
     const long long grid = (long long)h->Nr * h->Nt;
     {   // image -> log-polar samples x e^rho into F (aliases G)
         lpk::Gather g{};
-        g.src = h->d_fc; g.src_z = (long long)N * N; g.src_span = 0; g.W = N; g.H = N;
+        g.src = w.fc; g.src_z = (long long)N * N; g.src_span = 0; g.W = N; g.H = N;
         g.coord = h->d_cmapF; g.c_span = grid; g.c_row = h->Nt; g.c_ro = 0; g.c_co = 0;
         g.scale = h->d_erho; g.s_row = h->Nt;
-        g.out = h->d_G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
+        g.out = w.G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxF;
         lpk::launch_gather(g, h->d_tilesF, h->ntilesF, 0, s);
     }
     mark(3);
     if (h->stop_after == 3) return check_launch(h, "lp_fwd launch");
-    lpk::launch_rows_fwd(rp, h->d_G, h->d_rangeF, s);
+    lpk::launch_rows_fwd(rp, w.G, h->d_rangeF, s);
     mark(4);
     if (h->stop_after == 4) return check_launch(h, "lp_fwd launch");
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
     mark(5);
     if (h->stop_after == 5) return check_launch(h, "lp_fwd launch");
     // only the columns the polar gather reads (all of them when debugging stage 6)
-    lpk::launch_rows_inv(rp, h->d_G, h->stop_after ? nullptr : h->d_wrangeF, s);
+    lpk::launch_rows_inv(rp, w.G, h->stop_after ? nullptr : h->d_wrangeF, s);
     mark(6);
     if (h->stop_after == 6) return check_launch(h, "lp_fwd launch");
     {   // log-polar -> polar for the kept angles, cropped detector
         lpk::Gather g{};
-        g.src = h->d_G; g.src_z = grid; g.src_span = (long long)ns0 * grid; g.W = h->Nt; g.H = h->Nr;
+        g.src = w.G; g.src_z = grid; g.src_span = (long long)ns0 * grid; g.W = h->Nt; g.H = h->Nr;
         g.coord = h->d_p2lp; g.c_span = 0; g.c_row = h->os * N; g.c_ro = 0; g.c_co = N / 2 - h->cor;
         g.scale = nullptr; g.s_row = 0;
         g.out = R; g.o_z = (long long)h->P0 * h->N0; g.o_span = 0; g.o_row = h->N0;
@@ -705,6 +710,15 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     }
     mark(7);
     return check_launch(h, "lp_fwd launch");
+}
+
+int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
+    if (!h) return LP_ERR_ARG;
+    if (!h->has_fwd) return h->fail(LP_ERR_ARG, "forward tables not initialised (lp_init_fwd)");
+    if (!R || !f) return h->fail(LP_ERR_ARG, "null data pointer");
+    if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
+    DeviceGuard guard(h->device);
+    return fwd_chain(h, R, f, ns0, (hipStream_t)stream, FwdScratch{h->d_fc, h->d_tmp, h->d_G, h->d_S}, true);
 }
 
 int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
@@ -884,6 +898,15 @@ void lp_destroy(lp_handle* h) { delete h; }
 const char* lp_last_error(const lp_handle* h) {
     if (!h) return g_create_err.c_str();
     return h->err.c_str();
+}
+
+int lp_stream_bench(int device, float* dst, const float* src, size_t n_floats, int kind, int blocks, void* stream) {
+    if (!dst || (kind != 2 && !src) || n_floats % 4 || kind < 0 || kind > 2) return LP_ERR_ARG;
+    DeviceGuard guard(device);
+    if (!guard.ok) return LP_ERR_DEVICE;
+    (void)hipGetLastError();
+    if (lpk::launch_stream(kind, dst, src, n_floats / 4, blocks, (hipStream_t)stream)) return LP_ERR_ARG;
+    return hipGetLastError() == hipSuccess ? LP_OK : LP_ERR_DEVICE;
 }
 
 int lp_mem_info(int device, size_t* free_bytes, size_t* total_bytes) {

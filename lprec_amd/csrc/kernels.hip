@@ -2002,4 +2002,52 @@ void launch_sino_rows_T(const float* Rc, float* R, int nz, const SinoPrep& sp, b
                      (int)cubic);
 }
 
+// ---------------------------------------------------------------------------
+// Streaming kernels that measure the HBM ceiling the pipeline's passes are
+// compared with: 16-byte loads and stores, UL independent loads in flight per
+// lane, grid-stride over n4 float4.  kind 0 copies (read + write), 1 reads
+// (the per-lane sums land in dst[thread], so the loads cannot be dropped),
+// 2 writes.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int KIND>
+__global__ void __launch_bounds__(256) k_stream(v4f* __restrict__ dst, const v4f* __restrict__ src, size_t n4) {
+    constexpr int UL = 4;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    const v4f one = {1.f, 1.f, 1.f, 1.f};
+    for (; i + (UL - 1) * stride < n4; i += UL * stride) {
+        v4f v[UL];
+        if (KIND != 2) {
+#pragma unroll
+            for (int u = 0; u < UL; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+        }
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            if (KIND == 0) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+            else if (KIND == 1) acc += v[u];
+            else __builtin_nontemporal_store(one, dst + i + u * stride);
+        }
+    }
+    for (; i < n4; i += stride) {
+        if (KIND == 0) dst[i] = src[i];
+        else if (KIND == 1) acc += src[i];
+        else dst[i] = one;
+    }
+    if (KIND == 1) dst[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+int launch_stream(int kind, float* dst, const float* src, size_t n4, int blocks, hipStream_t s) {
+    if (blocks < 1) blocks = 2048;
+    v4f* d = reinterpret_cast<v4f*>(dst);
+    const v4f* a = reinterpret_cast<const v4f*>(src);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_stream<0>, dim3(blocks), dim3(256), 0, s, d, a, n4); return 0;
+        case 1: hipLaunchKernelGGL(k_stream<1>, dim3(blocks), dim3(256), 0, s, d, a, n4); return 0;
+        case 2: hipLaunchKernelGGL(k_stream<2>, dim3(blocks), dim3(256), 0, s, d, a, n4); return 0;
+    }
+    return -1;
+}
+
 }  // namespace lpk

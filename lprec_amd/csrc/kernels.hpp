@@ -136,6 +136,10 @@ void launch_sino_rows_T(const float* Rc, float* R, int nz, const SinoPrep& sp, b
 // p[0..n) = 0 (a kernel, so that it may be captured in a graph)
 void launch_zero(float* p, size_t n, hipStream_t s);
 
+// HBM streaming kernels (kind 0 copy, 1 read, 2 write) over n4 float4 with
+// `blocks` blocks of 256 lanes; for kind 1 dst must hold blocks * 256 float4.
+int launch_stream(int kind, float* dst, const float* src, size_t n4, int blocks, hipStream_t s);
+
 size_t rows_lds_bytes(int ldm, int R);
 size_t cols_lds_bytes(int M);
 size_t sino_lds_bytes(int Lh, int N);
