@@ -63,6 +63,15 @@ int lp_init_adj(lp_handle* h, const int32_t* adjparsi, const float* adjparamsf);
  * (lpRgpu.cu:184-193): f [ns0][N0][N0] -> R [ns0][Nproj0][N0], device memory.
  * Requires 1 <= ns0 <= Nslices. */
 int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream);
+/* lp_fwd with no allocation of any kind: no reference counterpart.  For
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same operator and result as
+ * lp_fwd, but the call's device scratch (prefiltered image, log-polar grids,
+ * half spectra: 52 N^2 bytes per slice) comes from one static 64 MiB array
+ * per GPU instead of the handle's buffers; larger calls (e.g. one slice at
+ * N > 1024) return LP_ERR_ARG.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other handles) race. */
+int lp_fwd_noalloc(lp_handle* h, float* R, const float* f, int ns0, void* stream);
 /* Replaces lpRgpu::execAdjManyPtr(size_t fptr, size_t Rptr, int Nslices0, int gpu)
  * (lpRgpu.cu:196-207): R [ns0][Nproj0][N0] -> f [ns0][N0][N0]; applies the FBP
  * filter when the adjoint tables carry one. */

@@ -25,7 +25,7 @@ EXPORTS = (
     "lp_pre_create", "lp_pre_sizes", "lp_pre_copy", "lp_pre_destroy", "lp_pre_error",
     "lp_create", "lp_init_fwd", "lp_init_adj", "lp_fwd", "lp_adj", "lp_fwd_T", "lp_adj_T", "lp_destroy",
     "lp_last_error", "lp_mem_info", "lp_info", "lp_set_tuning", "lp_get_tuning",
-    "lp_profile", "lp_profile_read", "lp_debug_stop", "lp_debug_copy", "lp_stream_bench",
+    "lp_profile", "lp_profile_read", "lp_debug_stop", "lp_debug_copy", "lp_stream_bench", "lp_fwd_noalloc",
 )
 
 
@@ -59,6 +59,7 @@ def load():
         "lp_init_fwd": (i32, [vp, vp, vp]),
         "lp_init_adj": (i32, [vp, vp, vp]),
         "lp_fwd": (i32, [vp, vp, vp, i32, vp]),
+        "lp_fwd_noalloc": (i32, [vp, vp, vp, i32, vp]),
         "lp_adj": (i32, [vp, vp, vp, i32, vp]),
         "lp_fwd_T": (i32, [vp, vp, vp, i32, vp]),
         "lp_adj_T": (i32, [vp, vp, vp, i32, vp]),

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -329,6 +330,36 @@ int pick_rows(lp_handle* h) {
     }
     return R;
 }
+
+// Scratch of lp_fwd_noalloc: the device buffers of one forward call, carved
+// from one statically allocated array per GPU so that the call allocates
+// nothing.  Each GPU the library runs on holds its own copy until the process
+// ends.  In host code `scratch` names a host-side stand-in: its device address
+// comes from hipGetSymbolAddress, once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current).
+int scratch_address(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return LP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return LP_ERR_DEVICE;
+        }
+        g_scratch_dev[device] = p;
+    }
+    *out = g_scratch_dev[device];
+    return LP_OK;
+}
+
+inline size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -719,6 +750,32 @@ int lp_fwd(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
     DeviceGuard guard(h->device);
     return fwd_chain(h, R, f, ns0, (hipStream_t)stream, FwdScratch{h->d_fc, h->d_tmp, h->d_G, h->d_S}, true);
+}
+
+int lp_fwd_noalloc(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
+    if (!h) return LP_ERR_ARG;
+    if (!h->has_fwd) return h->fail(LP_ERR_ARG, "forward tables not initialised (lp_init_fwd)");
+    if (!R || !f) return h->fail(LP_ERR_ARG, "null data pointer");
+    if (ns0 < 1 || ns0 > h->Ns) return h->fail(LP_ERR_ARG, "ns0 must be in [1, Nslices]");
+    const size_t grid = (size_t)h->Nr * h->Nt;
+    const size_t b_fc = round256((size_t)ns0 * h->N * h->N * sizeof(float));
+    const size_t b_G = round256(3 * (size_t)ns0 * grid * sizeof(float));
+    const size_t b_S = round256(3 * (size_t)ns0 * h->h * h->Nr * sizeof(float2));
+    // the row-sweep output (N x N per slice) reuses G, which is 6 N^2 per slice
+    if (b_fc + b_G + b_S > kNoAllocScratchBytes)
+        return h->fail(LP_ERR_ARG, "lp_fwd_noalloc: " + std::to_string(ns0) + " slice(s) at N = " +
+                                       std::to_string(h->N) + " need " + std::to_string(b_fc + b_G + b_S) +
+                                       " bytes of scratch; the static scratch holds " +
+                                       std::to_string(kNoAllocScratchBytes) + " bytes");
+    DeviceGuard guard(h->device);
+    if (!guard.ok) return h->fail(LP_ERR_DEVICE, "hipSetDevice failed");
+    void* base = nullptr;
+    if (int rc = scratch_address(h->device, &base)) return h->fail(rc, "hipGetSymbolAddress(scratch) failed");
+    unsigned char* p = static_cast<unsigned char*>(base);
+    float* fc = reinterpret_cast<float*>(p);
+    float* G = reinterpret_cast<float*>(p + b_fc);
+    float2* S = reinterpret_cast<float2*>(p + b_fc + b_G);
+    return fwd_chain(h, R, f, ns0, (hipStream_t)stream, FwdScratch{fc, G, G, S}, false);
 }
 
 int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
