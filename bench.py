@@ -24,10 +24,13 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 METRIC = "fwd+adj slices/s at N=2048,1500 proj: batched vs plain GPU; % of HBM roofline"
 
 
-def stage_bytes(g):
+def stage_bytes(g, fused_polar=False):
     """Algorithmic HBM bytes per slice of each stage (one launch covers all
     slices of a batch and all three spans).  L = 4*Nrho*Ntheta bytes is one
-    real log-polar grid; the half spectra S of a span also occupy L bytes."""
+    real log-polar grid; the half spectra S of a span also occupy L bytes.
+    With the forward operator's inverse row pass and polar gather fused, the
+    fused kernel is timed as rows_inv (S read, R written) and polar_out is
+    empty."""
     N0, N, P0, P = g["N0"], g["N"], g["Nproj0"], g["Nproj"]
     L = 4 * g["Nrho"] * g["Ntheta"]
     fwd = {
@@ -39,6 +42,9 @@ def stage_bytes(g):
         "rows_inv": 3 * 2 * L,                     # S read, G written per span
         "polar_out": 3 * L // 2 + 4 * N0 * P0,     # half of each G (|theta| < beta/2) read, R written
     }
+    if fused_polar:
+        fwd["rows_inv"] = 3 * L + 4 * N0 * P0        # S read per span, R written
+        fwd["polar_out"] = 0
     adj = {
         # the cubic path writes only the kept angle rows of the stuffed
         # sinogram (P0 of P = os P0); the column sweep reads them and writes all P
@@ -447,7 +453,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-kernel HBM ceiling")
-    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16, boxcap)")
+    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16, boxcap, fuse)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--pmc-summary", default="", metavar="DIR",
@@ -510,7 +516,7 @@ def main():
         return
     N0, P0, ns = args.N, args.nproj, args.slices
     cor = N0 / 2
-    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5}
+    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6}
     sched = {}
     for kv in filter(None, args.tune.split(",")):
         k, v = kv.split("=")
@@ -566,12 +572,13 @@ def main():
         plain = nplain / (time.perf_counter() - t1)
 
     # roofline of the dominant stage (largest summed duration)
-    sb_fwd, sb_adj = stage_bytes(g)
+    sb_fwd, sb_adj = stage_bytes(g, fused_polar=fwd_ms.get("polar_out", 1.0) == 0.0)
     stages = {}
     for name, ms in fwd_ms.items():
         stages["fwd." + name] = (ms / max(nf, 1), sb_fwd[name] * ns)
     for name, ms in adj_ms.items():
         stages["adj." + name] = (ms / max(na, 1), sb_adj[name] * ns)
+    stages = {k: v for k, v in stages.items() if v[1] > 0}
     dom = max(stages, key=lambda k: stages[k][0])
     dom_ms, dom_bytes = stages[dom]
     achieved = dom_bytes / (dom_ms * 1e-3)
@@ -601,10 +608,14 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "alg_bytes_per_launch": int(dom_bytes), "avg_launch_ms": round(dom_ms, 4),
                      "traffic_read_write": list(table[dom]) if traffic else None,
-                     "measured_ceiling": ceiling},
+                     "measured_ceiling": ceiling,
+                     "frac_of_copy_ceiling": (round(achieved / (ceiling["copy_GBps"] * 1e9), 4)
+                                              if ceiling else None)},
         "pipeline_roofline": {"B_alg_bytes_per_slice": int(b_alg),
                               "achieved_GBps": round(b_alg * value / 1e9, 1),
-                              "frac": round(b_alg * value / HBM_PEAK, 4)},
+                              "frac": round(b_alg * value / HBM_PEAK, 4),
+                              "frac_of_copy_ceiling": (round(b_alg * value / (ceiling["copy_GBps"] * 1e9), 4)
+                                                       if ceiling else None)},
         "plain_path": ({"value": round(plain, 2), "unit": "slices/s", "batched_over_plain": round(value / plain, 3)}
                        if plain else None),
         "stage_ms_per_step": {k: round(v[0], 4) for k, v in stages.items()},

@@ -114,7 +114,10 @@ int lp_info(const lp_handle* h, int64_t* info);
  * kernels where the length is 16*k*16 with k in {1,2,4,8,16}, 0 = the
  * mixed-radix LDS kernels everywhere), key 5 = LDS budget of a gather block
  * in floats (1024..32768; tiles are split until their source box fits; only
- * before lp_init_fwd / lp_init_adj).  Returns LP_ERR_ARG for bad values. */
+ * before lp_init_fwd / lp_init_adj), key 6 = forward operator's inverse row
+ * pass and polar gather fused into one kernel (1) or run as two (0, the
+ * default); the fused kernel applies with 16 rows per row-FFT block and the
+ * radix-16 kernels.  Returns LP_ERR_ARG for bad values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 

@@ -60,7 +60,11 @@ struct lp_handle {
     int N = 0, N0 = 0, Nt = 0, Nr = 0, P = 0, P0 = 0, Ns = 0, cor = 0, os = 1, h = 0;
     bool cubic = true;
     bool has_fwd = false, has_adj = false, has_filter = false;
-    int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1;
+    // fuse_polar: off by default -- the fused kernel's 16-row windows start at
+    // rows 13 b - 1, so each reads two 128-byte lines per spectrum column
+    // where the separate pass reads one, and it measured no faster than the
+    // separate pair (1.91 vs 1.90 ms per 32 slices at N = 2048)
+    int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1, fuse_polar = 0;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -83,6 +87,16 @@ struct lp_handle {
     int2* d_rangeA = nullptr;
     int2* d_wrangeF = nullptr;   // theta columns of G each (span, row) the polar gather reads
     int2* d_wrangeA = nullptr;   // ... and the Cartesian gather reads
+    // the fused inverse row pass + polar gather: samples per (span, band of 13 base rows)
+    float2* d_pcoord = nullptr;
+    int* d_poff = nullptr;
+    int* d_pband = nullptr;
+    int npband = 0;
+    // rows of the log-polar grids the convolution passes may skip (null: none)
+    int2* d_inrowsF = nullptr;            // [3] rows [lo, hi) where the gathered grid can be non-zero
+    int2* d_inrowsA = nullptr;
+    unsigned char* d_outrowsF = nullptr;  // [3][Nrho] rows the output gather reads
+    unsigned char* d_outrowsA = nullptr;
     // LDS-tiled gathers: tile lists and the largest source box of each
     lpk::GTile* d_tilesF = nullptr;   // image -> LP (per span)
     lpk::GTile* d_tilesP = nullptr;   // LP -> polar (per span)
@@ -186,6 +200,10 @@ int check_launch(lp_handle* h, const char* what) {
 }
 
 inline bool valid2(float2 v) { return std::isfinite(v.x) && std::isfinite(v.y); }
+
+// base rows per band of the fused inverse row pass + polar gather (rows per
+// block 16, minus the 3 extra rows of the cubic taps)
+constexpr int kFuseBase = 13;
 
 // Source box of one output tile: every cubic tap (floor(x)-1 .. floor(x)+2)
 // of every valid point, in unwrapped source coordinates.
@@ -296,6 +314,38 @@ std::vector<int2> row_ranges(const std::vector<float2>& cmap, int Nr, int Nt) {
             r[k * Nr + i] = make_int2(lo < hi ? lo : 0, lo < hi ? hi : 0);
         }
     return r;
+}
+
+// Per span, the rows [lo, hi) outside which a gathered grid is empty (from
+// the per-row sampled ranges), widened to multiples of 128 (the column pass
+// skips whole 128-row chunks).  Empty when every span needs all rows.
+std::vector<int2> data_rows(const std::vector<int2>& rng, int Nr) {
+    std::vector<int2> out(3);
+    bool all = true;
+    for (int k = 0; k < 3; ++k) {
+        int lo = Nr, hi = 0;
+        for (int r = 0; r < Nr; ++r)
+            if (rng[(size_t)k * Nr + r].x < rng[(size_t)k * Nr + r].y) { lo = std::min(lo, r); hi = r + 1; }
+        if (lo >= hi) lo = hi = 0;
+        lo = lo / 128 * 128;
+        hi = std::min(Nr, (hi + 127) / 128 * 128);
+        out[k] = make_int2(lo, hi);
+        if (lo != 0 || hi != Nr) all = false;
+    }
+    if (all) out.clear();
+    return out;
+}
+
+// Rows each output gather reads (non-empty read ranges); empty when all are read.
+std::vector<unsigned char> read_rows(const std::vector<int2>& wrng) {
+    std::vector<unsigned char> out(wrng.size());
+    bool all = true;
+    for (size_t i = 0; i < wrng.size(); ++i) {
+        out[i] = wrng[i].x < wrng[i].y;
+        if (!out[i]) all = false;
+    }
+    if (all) out.clear();
+    return out;
 }
 
 // kernel spectrum [Nrho][H] interleaved (wire format) -> transposed [H][Nrho]
@@ -520,10 +570,46 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
     std::vector<int2> wrng = box_row_ranges(tP, Nr, Nt);
     if ((rc = h->upload(&h->d_tilesF, tF)) || (rc = h->upload(&h->d_tilesP, tP))) return rc;
     if ((rc = h->upload(&h->d_wrangeF, wrng))) return rc;
+    {
+        std::vector<int2> in_rows = data_rows(rng, Nr);
+        std::vector<unsigned char> out_rows = read_rows(wrng);
+        if (!in_rows.empty() && (rc = h->upload(&h->d_inrowsF, in_rows))) return rc;
+        if (!out_rows.empty() && (rc = h->upload(&h->d_outrowsF, out_rows))) return rc;
+    }
     if ((rc = h->upload(&h->d_cmapF, cmap)) || (rc = h->upload(&h->d_rangeF, rng)) || (rc = h->upload(&h->d_p2lp, p2lp)) ||
         (rc = h->upload(&h->d_spanrow, spanrow)) || (rc = h->upload(&h->d_fzF, fzT)) ||
         (rc = h->alloc(&h->d_fc, (size_t)h->Ns * N * N)))
         return rc;
+    {
+        // polar samples of the kept angles by (span, band of kFuseBase base
+        // rows): their table coordinates and output offsets a * N0 + x
+        const int P0 = h->P0, lb = N / 2 - h->cor, os = h->os, N0 = h->N0;
+        const int nband = (Nr + kFuseBase - 1) / kFuseBase;
+        std::vector<int> cnt(3 * (size_t)(nband + 1), 0);
+        std::vector<int> key((size_t)P0 * N0);
+        for (int a = 0; a < P0; ++a)
+            for (int x = 0; x < N0; ++x) {
+                const float2 c = p2lp[(size_t)a * os * N + x + lb];
+                const long long iy = (long long)std::floor(c.y);
+                const int iyw = (int)(((iy % Nr) + Nr) % Nr);
+                const int k = spanrow[(size_t)a * os];
+                key[(size_t)a * N0 + x] = k * (nband + 1) + iyw / kFuseBase;
+                cnt[key[(size_t)a * N0 + x] + 1]++;
+            }
+        for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
+        std::vector<int> beg(cnt);
+        std::vector<float2> pc((size_t)P0 * N0);
+        std::vector<int> po((size_t)P0 * N0);
+        for (int a = 0; a < P0; ++a)
+            for (int x = 0; x < N0; ++x) {
+                const size_t at = (size_t)beg[key[(size_t)a * N0 + x]]++;
+                pc[at] = p2lp[(size_t)a * os * N + x + lb];
+                po[at] = a * N0 + x;
+            }
+        h->npband = nband;
+        if ((rc = h->upload(&h->d_pcoord, pc)) || (rc = h->upload(&h->d_poff, po)) || (rc = h->upload(&h->d_pband, cnt)))
+            return rc;
+    }
     h->has_fwd = true;
     return LP_OK;
 }
@@ -630,6 +716,12 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
     h->ntilesC = (int)tC.size();
     std::vector<int2> wrng = box_row_ranges(tC, Nr, Nt);
     if ((rc = h->upload(&h->d_wrangeA, wrng))) return rc;
+    {
+        std::vector<int2> in_rows = data_rows(rng, Nr);
+        std::vector<unsigned char> out_rows = read_rows(wrng);
+        if (!in_rows.empty() && (rc = h->upload(&h->d_inrowsA, in_rows))) return rc;
+        if (!out_rows.empty() && (rc = h->upload(&h->d_outrowsA, out_rows))) return rc;
+    }
     if ((rc = h->upload(&h->d_tilesA, tA)) || (rc = h->upload(&h->d_tilesC, tC))) return rc;
     if ((rc = h->upload(&h->d_cmapA, cmap)) || (rc = h->upload(&h->d_rangeA, rng)) || (rc = h->upload(&h->d_cmapC, cmapC)) ||
         (rc = h->upload(&h->d_fzA, fzT)) || (rc = h->alloc(&h->d_Rc, (size_t)h->Ns * h->P * N)))
@@ -680,6 +772,8 @@ static lpk::RowPass row_pass(lp_handle* h, int nz) {
     rp.S = h->d_S;
     rp.conj = 0;
     rp.fft16 = h->fft16;
+    rp.in_rows = nullptr;
+    rp.out_rows = nullptr;
     return rp;
 }
 
@@ -708,6 +802,10 @@ static int fwd_chain(lp_handle* h, float* R, const float* f, int ns0, hipStream_
     if (h->stop_after == 2) return check_launch(h, "lp_fwd launch");
     lpk::RowPass rp = row_pass(h, ns0);
     rp.S = w.S;
+    if (!h->stop_after) {   // (stage-wise tests read every row)
+        rp.in_rows = h->d_inrowsF;
+        rp.out_rows = h->d_outrowsF;
+    }
     const long long grid = (long long)h->Nr * h->Nt;
     {   // image -> log-polar samples x e^rho into F (aliases G)
         lpk::Gather g{};
@@ -726,6 +824,18 @@ static int fwd_chain(lp_handle* h, float* R, const float* f, int ns0, hipStream_
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
     mark(5);
     if (h->stop_after == 5) return check_launch(h, "lp_fwd launch");
+    if (!h->stop_after && h->fuse_polar && h->R == kFuseBase + 3) {
+        // inverse row pass and polar gather in one kernel (G stays on chip)
+        lpk::PolarOut po;
+        po.out = R; po.o_z = (long long)h->P0 * h->N0;
+        po.coord = h->d_pcoord; po.off = h->d_poff; po.band_beg = h->d_pband; po.nband = h->npband;
+        po.cubic = h->cubic;
+        if (lpk::launch_rows_inv_polar(rp, po, s)) {
+            mark(6);
+            mark(7);
+            return check_launch(h, "lp_fwd launch");
+        }
+    }
     // only the columns the polar gather reads (all of them when debugging stage 6)
     lpk::launch_rows_inv(rp, w.G, h->stop_after ? nullptr : h->d_wrangeF, s);
     mark(6);
@@ -806,6 +916,10 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     mark(2);
     if (h->stop_after == 2) return check_launch(h, "lp_adj launch");
     lpk::RowPass rp = row_pass(h, ns0);
+    if (!h->stop_after) {   // (stage-wise tests read every row)
+        rp.in_rows = h->d_inrowsA;
+        rp.out_rows = h->d_outrowsA;
+    }
     lpk::Src src{h->d_Rc, N, h->P, (long long)h->P * N};
     const long long grid = (long long)h->Nr * h->Nt;
     {   // sinogram -> log-polar samples into F (aliases G)
@@ -1011,6 +1125,10 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "FFT kernel family must be 0 or 1");
             h->fft16 = value;
             return LP_OK;
+        case 6:
+            if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "polar fusion must be 0 or 1");
+            h->fuse_polar = value;
+            return LP_OK;
         case 5:
             if (h->has_fwd || h->has_adj)
                 return h->fail(LP_ERR_ARG, "the gather box budget is fixed once lp_init_fwd / lp_init_adj ran");
@@ -1085,6 +1203,7 @@ int lp_get_tuning(const lp_handle* h, int key, int* value) {
         case 3: *value = h->gzgroup; return LP_OK;
         case 4: *value = h->fft16; return LP_OK;
         case 5: *value = h->box_cap; return LP_OK;
+        case 6: *value = h->fuse_polar; return LP_OK;
     }
     return LP_ERR_ARG;
 }

@@ -830,7 +830,10 @@ __global__ void k_cols(RowPass rp, FftPlan planM, const float2* __restrict__ twM
     const float2* fzh = fzT + (size_t)h * M;
     for (int z = z0; z < z1; ++z) {
         float2* col = rp.S + (((size_t)span * rp.nz + z) * h + f) * (size_t)M;
-        stage_in(M, [&](int i) { return col[i]; }, [&](int i, float2 v) { smc[lidx(i)] = v; });
+        const int2 ir = rp.in_rows ? rp.in_rows[span] : make_int2(0, M);
+        stage_in(
+            M, [&](int i) { return (i >= ir.x && i < ir.y) ? col[i] : make_float2(0.f, 0.f); },
+            [&](int i, float2 v) { smc[lidx(i)] = v; });
         __syncthreads();
         fft_lds<EPT>(smc, M, 1, planM, twM, -1);
         if (f == 0) {
@@ -850,7 +853,9 @@ __global__ void k_cols(RowPass rp, FftPlan planM, const float2* __restrict__ twM
         }
         __syncthreads();
         fft_lds<EPT>(smc, M, 1, planM, twM, +1);
-        for (int i = threadIdx.x; i < M; i += blockDim.x) col[i] = smc[lidx(i)];
+        const unsigned char* om = rp.out_rows ? rp.out_rows + (size_t)span * M : nullptr;
+        for (int i = threadIdx.x; i < M; i += blockDim.x)
+            if (!om || om[i]) col[i] = smc[lidx(i)];
         __syncthreads();
     }
 }
@@ -948,12 +953,24 @@ __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __r
     const int z0 = zg * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     constexpr int tstep3 = M / (16 * MID * 16);  // last stage: Ns = 16 MID, radix 16 (= 1)
+    const int2 ir = rp.in_rows ? rp.in_rows[span] : make_int2(0, M);
+    unsigned smask = 0xffffu;   // stored rows among li + q T16, read once
+    if (rp.out_rows) {
+        const unsigned char* om = rp.out_rows + (size_t)span * M;
+        smask = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) smask |= om[li + q * T16] ? (1u << q) : 0u;
+    }
     for (int z = z0; z < z1; ++z) {
         float2* col = rp.S + (((size_t)span * rp.nz + z) * h + f) * (size_t)M;
         float2 v[16];
-        // ---- forward stage 0: radix 16, Ns = 1, straight from HBM
+        // ---- forward stage 0: radix 16, Ns = 1, straight from HBM (rows
+        // without data are zeros)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = col[li + q * T16];
+        for (int q = 0; q < 16; ++q) {
+            const int i = li + q * T16;
+            v[q] = (i >= ir.x && i < ir.y) ? col[i] : make_float2(0.f, 0.f);
+        }
         dft16(v, -1);
         if (worker) {
 #pragma unroll
@@ -984,7 +1001,8 @@ __global__ void __launch_bounds__(256, 2) k_cols16(RowPass rp, const float2* __r
         dft16(v, +1);
         if (worker) {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) col[li + q * T16] = v[q];
+            for (int q = 0; q < 16; ++q)
+                if (smask & (1u << q)) col[li + q * T16] = v[q];
         }
         __syncthreads();
     }
@@ -1044,12 +1062,28 @@ __global__ void __launch_bounds__(256, 2) k_cols4096(RowPass rp, const float2* _
     const int z0 = zg * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
     const int wv = li >> 6, ln = li & 63;
+    // 128-row chunks of the column that carry data (a multiple of 128 bounds
+    // the rows with data); the others stay zero in `pre`
+    const int2 ir = rp.in_rows ? rp.in_rows[span] : make_int2(0, M);
+    const int c_lo = ir.x >> 7, c_hi = (ir.y + 127) >> 7;
+    // rows of this thread's 16 outputs (li + q 256) that are stored: read once,
+    // before the stream starts, so that no other global load joins it
+    unsigned smask = 0xffffu;
+    if (rp.out_rows) {
+        const unsigned char* om = rp.out_rows + (size_t)span * M;
+        smask = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) smask |= om[li + q * T16] ? (1u << q) : 0u;
+    }
+    for (int i = li; i < M; i += T16)
+        if ((i >> 7) < c_lo || (i >> 7) >= c_hi) pre[i] = make_float2(0.f, 0.f);
     auto prefetch = [&](int z) {
         const float2* col = rp.S + (((size_t)span * rp.nz + z) * h + f) * (size_t)M;
 #pragma unroll
         for (int c = 0; c < M / 128 / 4; ++c) {
             const int cc = wv + 4 * c;
-            __builtin_amdgcn_global_load_lds(col + cc * 128 + ln * 2, pre + cc * 128, 16, 0, 0);
+            if (cc >= c_lo && cc < c_hi)
+                __builtin_amdgcn_global_load_lds(col + cc * 128 + ln * 2, pre + cc * 128, 16, 0, 0);
         }
     };
     prefetch(z0);
@@ -1117,7 +1151,8 @@ __global__ void __launch_bounds__(256, 2) k_cols4096(RowPass rp, const float2* _
         twiddle16_w(v, wl, +1);
         dft16(v, +1);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) col[li + q * T16] = v[q];
+        for (int q = 0; q < 16; ++q)
+            if (smask & (1u << q)) col[li + q * T16] = v[q];
         lds_barrier();
     }
 }
@@ -1265,6 +1300,11 @@ __global__ void __launch_bounds__(1024) k_rows16_fwd(RowPass rp, const float* __
     const int row0 = (lb % nrb) * R, span = lb / (nrb * nzg);
     const int z0 = ((lb / nrb) % nzg) * rp.zgroup;
     const int z1 = min(z0 + rp.zgroup, rp.nz);
+    if (rp.in_rows) {
+        // no data in these rows: the column pass reads them as zeros
+        const int2 ir = rp.in_rows[span];
+        if (row0 + R <= ir.x || row0 >= ir.y) return;
+    }
     const int2 lr = range ? range[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
     float2* rowbuf = smc + r * RS;
     // Up to H = 2048 the block's twiddles (twh[0, H/2), twN[0, H/2]) fit in
@@ -1357,6 +1397,9 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
     // a range of float2 slots
     const int2 wr = wrange ? wrange[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
     const int e0 = max(wr.x, 0) >> 1, e1 = min((wr.y + 1) >> 1, H);
+    // no row of the block is read by the output gather (the column pass did
+    // not store these rows either)
+    if (rp.out_rows && !__syncthreads_or(e0 < e1)) return;
     float2* rowbuf = smc + r * RS;
     // (k_rows16_fwd stages its twiddles in LDS; the same staging measured
     // 4 % slower here, so this pass reads them from the tables)
@@ -1406,6 +1449,107 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
         row_sync<WAVE>();
         float2* out = reinterpret_cast<float2*>(G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt));
         for (int e = e0 + li; e < e1; e += T16) out[e] = rowbuf[lidx(e)];
+        lds_barrier();   // the next slice's staging overwrites every row (stores drain meanwhile)
+    }
+}
+
+// Inverse row pass fused with the log-polar -> polar gather of the forward
+// operator.  A block owns a band of `nbase` = R - 3 base rows of one span:
+// it inverse-transforms the R rows [c0, c0 + R) (c0 = band * nbase - 1, mod
+// Nrho), which hold every cubic tap of the polar samples whose floor(rho)
+// lies in the band, keeps the real rows in LDS and evaluates those samples
+// from there, so the convolved grid G never goes through HBM.  At fixed angle
+// rho' grows with the detector coordinate, so a band's samples are runs of
+// consecutive detector cells per angle (listed on the host in that order).
+template <int MID>
+__global__ void __launch_bounds__(1024) k_rows16_inv_polar(RowPass rp, PolarOut po) {
+    extern __shared__ float2 smc[];
+    constexpr int H = 256 * MID, T16 = 16 * MID;
+    constexpr bool WAVE = T16 <= 64;
+    const int Nt = rp.Ntheta, R = rp.R, Nr = rp.Nrho;
+    const int RS = H + H / 16 + 32 / R;
+    const int lgR = __builtin_ctz((unsigned)R);
+    const int nbase = R - 3;
+    const int r = threadIdx.x / T16, li = threadIdx.x - r * T16;
+    const int nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int band = lb % po.nband, span = lb / (po.nband * nzg);
+    const int z0 = ((lb / po.nband) % nzg) * rp.zgroup;
+    const int z1 = min(z0 + rp.zgroup, rp.nz);
+    const int* bb = po.band_beg + (size_t)span * (po.nband + 1) + band;
+    const int beg = bb[0], end = bb[1];
+    if (beg >= end) return;   // no polar sample has its base row here
+    const int c0 = band * nbase - 1;
+    float2* rowbuf = smc + r * RS;
+    const float* rowsf = reinterpret_cast<const float*>(smc);
+    const int fstride = 2 * RS;   // floats per LDS row
+    const float2* twh = rp.twh;
+    const float2* twN = rp.twN;
+    for (int z = z0; z < z1; ++z) {
+        {
+            // transposed load of rows (c0 + rr) mod Nrho
+            const int rr = threadIdx.x & (R - 1), f0 = threadIdx.x >> lgR;
+            float2* dst = smc + rr * RS + lidx(f0);
+            const float2* in = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr;
+            const unsigned lane = (unsigned)(f0 * Nr + ((c0 + rr) & (Nr - 1))) * 8u;
+            float2 w[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) w[i] = ld_at(in + (size_t)(i * T16) * Nr, lane);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) dst[i * (T16 + T16 / 16)] = w[i];
+        }
+        lds_barrier();
+        c2r_pre_row<H, T16>(rowbuf, twN, li);
+        row_sync<WAVE>();
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
+        dft16(v, +1);
+        row_sync<WAVE>();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rowbuf[lidx(li * 16 + q)] = v[q];
+        row_sync<WAVE>();
+        if (MID > 1) row_mid_stage<MID, WAVE>(rowbuf, +1, twh, li);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = rowbuf[lidx(li + q * T16)];
+        twiddle16(v, twh, li, +1);
+        dft16(v, +1);
+        row_sync<WAVE>();
+        // the row's real samples, unpadded: theta = 2e, 2e + 1 in slot e
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rowbuf[li + q * T16] = v[q];
+        lds_barrier();   // every row is complete
+        float* Oz = po.out + (size_t)z * po.o_z;
+        for (int i = beg + (int)threadIdx.x; i < end; i += blockDim.x) {
+            const float2 c = po.coord[i];
+            const int off = po.off[i];
+            const float fx = floorf(c.x), fy = floorf(c.y);
+            const float ax = c.x - fx, ay = c.y - fy;
+            // local row of tap 0 (row floor(rho) - 1, mod Nrho): in [0, nbase)
+            const int l0 = ((int)fy - band * nbase) & (Nr - 1);
+            float val;
+            if (po.cubic) {
+                const int j0 = (int)fx - 1;
+                const float* q0 = rowsf + l0 * fstride;
+                float wx[4], wy[4];
+                bspline_w6(ax, wx);
+                bspline_w6(ay, wy);
+                const int ja = j0 & (Nt - 1), jb = (j0 + 1) & (Nt - 1), jc = (j0 + 2) & (Nt - 1), jd = (j0 + 3) & (Nt - 1);
+                val = 0.f;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const float* qr = q0 + b * fstride;
+                    val += wy[b] * (wx[0] * qr[ja] + wx[1] * qr[jb] + wx[2] * qr[jc] + wx[3] * qr[jd]);
+                }
+                val *= (1.0f / 36.0f);
+            } else {
+                const int ja = (int)fx & (Nt - 1), jb = ((int)fx + 1) & (Nt - 1);
+                const float* q1 = rowsf + (l0 + 1) * fstride;
+                const float* q2 = q1 + fstride;
+                val = (1.f - ay) * ((1.f - ax) * q1[ja] + ax * q1[jb]) + ay * ((1.f - ax) * q2[ja] + ax * q2[jb]);
+            }
+            Oz[off] = val;
+        }
         lds_barrier();   // the next slice's staging overwrites every row (stores drain meanwhile)
     }
 }
@@ -1591,6 +1735,17 @@ void launch_rows_inv(const RowPass& rp0, float* G, const int2* wrange, hipStream
     int ldm = rg.ldm;
     dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
     LP_DISPATCH_EPT(ept, k_rows_inv, grid, dim3(nt), rows_lds_bytes(ldm, rg.R), s, rg, G, wrange, ldm);
+}
+
+bool launch_rows_inv_polar(const RowPass& rp0, const PolarOut& po, hipStream_t s) {
+    if (rp0.R != 16 || (rp0.Ntheta & (rp0.Ntheta - 1)) || (rp0.Nrho & (rp0.Nrho - 1))) return false;
+    const RowPass rp = fit_zgroup(rp0, 3LL * po.nband);
+    const int mid = rows16_mid(rp);
+    if (!mid) return false;
+    dim3 grid(po.nband * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
+    dim3 block(rp.R * (rp.h / 16));
+    LP_DISPATCH_MID(mid, k_rows16_inv_polar, grid, block, rows16_lds(rp), s, rp, po);
+    return true;
 }
 
 // middle radix of M = 16 * MID * 16, or 0 when the register kernel does not apply

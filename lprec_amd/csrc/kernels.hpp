@@ -50,6 +50,14 @@ struct RowPass {
     float2* S;                // [3][nz][h][Nrho]
     int conj;                 // column pass multiplies by conj(fZ) (the transposed convolution)
     int fft16;                // use the radix-16 register kernels where the length allows
+    // Rows of the log-polar grids that carry data (null = all rows):
+    // in_rows[span] = [lo, hi), multiples of 128, outside which the gathered
+    // grid is zero -- the row pass writes no half-spectrum rows there and the
+    // column pass reads none (zeros); out_rows[span * Nrho + row] != 0 for the
+    // rows the output gather reads -- the column pass stores only those and
+    // the inverse row pass skips blocks with none.
+    const int2* in_rows;
+    const unsigned char* out_rows;
 };
 // R2C along theta of the gathered log-polar grids F [3][nz][Nrho][Ntheta];
 // range [3][Nrho]: [lo, hi) theta bounds of the non-empty points of each row
@@ -63,6 +71,20 @@ void launch_add_extras(float* F, int nz, int Ntheta, int Nrho, const Src& src, c
 // C2R along theta -> G [3][nz][Nrho][Ntheta]; only the theta columns
 // [lo, hi) of wrange [3][Nrho] are stored (null: whole rows)
 void launch_rows_inv(const RowPass& rp, float* G, const int2* wrange, hipStream_t s);
+// The inverse row pass fused with the log-polar -> polar gather (forward
+// operator): bands of R - 3 base rows per block (R = 16), the polar samples
+// of each (span, band) listed in band_beg order.
+struct PolarOut {
+    float* out;               // R [nz][Nproj0][N0]
+    long long o_z;            // slice stride of out
+    const float2* coord;      // per sample: (theta, rho) table coordinates
+    const int* off;           // per sample: offset a * N0 + x in a slice of out
+    const int* band_beg;      // [3][nband + 1] first sample of each (span, band)
+    int nband;
+    int cubic;
+};
+// false when the register kernels do not apply (the caller runs the unfused pair)
+bool launch_rows_inv_polar(const RowPass& rp, const PolarOut& po, hipStream_t s);
 // FFT along rho, x fZ, inverse, in place on S; fzT [h+1][Nrho] (scaled)
 void launch_cols(const RowPass& rp, const FftPlan& planM, const float2* twM, const float2* fzT, hipStream_t s);
 

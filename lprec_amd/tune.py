@@ -1,8 +1,9 @@
 """Schedule autotuner for one operator handle.
 
-The kernels expose five schedule knobs through ``lp_set_tuning`` (log-polar
+The kernels expose six schedule knobs through ``lp_set_tuning`` (log-polar
 rows per row-FFT block, slices per FFT block, prefilter column chunk, slices
-per gather block, FFT kernel family).  ``autotune`` times fwd+adj (and,
+per gather block, FFT kernel family, the forward operator's fused inverse row
+pass + polar gather).  ``autotune`` times fwd+adj (and,
 optionally, the transposes) on the handle's own stream with device events
 and walks the knobs one at a time (coordinate descent, two sweeps), keeping
 the fastest setting of each; settings the handle rejects (LDS or register
@@ -16,15 +17,16 @@ import torch
 
 from ._lib import LprecError
 
-KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5}
+KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6}
 # knobs that can change after the tables are built (boxcap is fixed at init)
-RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16")
+RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16", "fuse")
 SPACE = {
     "rows": (4, 8, 16),
     "zgroup": (1, 2, 4, 8),
     "gzgroup": (1, 2, 4, 8),
     "colchunk": (32, 64, 128),
     "fft16": (1, 0),
+    "fuse": (1, 0),
 }
 
 

@@ -272,3 +272,34 @@ def test_batch_of_32_slices_at_512_matches_mirror(torch_cuda):
     del lp
     release()
     assert ef <= TOL and ea <= TOL
+
+
+@pytest.mark.parametrize("N0,P0,ns,interp", [(2048, 1500, 32, "cubic"), (1024, 1024, 4, "cubic"),
+                                             (512, 400, 8, "cubic"), (256, 384, 4, "linear")],
+                         ids=["N2048_b32", "N1024", "N512", "N256_linear"])
+def test_fused_polar_output_matches_two_kernel_path(headline, torch_cuda, N0, P0, ns, interp):
+    """The forward operator's fused inverse row pass + polar gather (tuning
+    key 6 = 1) against the separate kernels (key 6 = 0, the default): the same
+    arithmetic on the same values (relative L2 <= 1e-6)."""
+    torch = torch_cuda
+    if N0 == 2048:
+        lp, _ = headline
+    else:
+        lp, _ = make(N0, P0, ns, interp=interp)
+    h = lp.clphandle[0]
+    f = torch.from_numpy(np.random.default_rng(N0).standard_normal((ns, N0, N0)).astype(np.float32)).cuda()
+    Rf = torch.zeros((ns, P0, N0), device="cuda")
+    Ru = torch.full((ns, P0, N0), 3.0, device="cuda")
+    try:
+        h.set_tuning(6, 1)
+        lp.fwdp(Rf, f, 0)
+        h.set_tuning(6, 0)
+        lp.fwdp(Ru, f, 0)
+    finally:
+        h.set_tuning(6, 0)
+    torch.cuda.synchronize()
+    a, b = Rf.cpu().numpy(), Ru.cpu().numpy()
+    e = rel(a, b)
+    print("N0 %d %s: fused vs two kernels relative %.2e, max abs %.2e" % (N0, interp, e, np.max(np.abs(a - b))),
+          flush=True)
+    assert e <= 1e-6
