@@ -60,11 +60,10 @@ def stage_bytes(g, fused_polar=False):
 
 
 def pipeline_bytes(g):
-    """B_alg per slice of fwd+adj (compulsory traffic of the fused design, cubic):
-    B_fwd = 4N0^2 + 16N^2 + 12L + 4N0P0, B_adj = 4N0P0 + 8NP0 + 8NP + 12L + 4N0^2."""
-    N0, N, P0, P = g["N0"], g["N"], g["Nproj0"], g["Nproj"]
-    L = 4 * g["Nrho"] * g["Ntheta"]
-    return (4 * N0 * N0 + 16 * N * N + 12 * L + 4 * N0 * P0) + (4 * N0 * P0 + 8 * N * P0 + 8 * N * P + 12 * L + 4 * N0 * N0)
+    """B_alg per slice of fwd+adj (lprec_amd.traffic: the compulsory traffic of
+    the fused design, cubic)."""
+    from lprec_amd import traffic
+    return traffic.b_alg(g)
 
 
 # Kernel name prefix -> stage of the pipeline (one fwd or adj call launches
@@ -294,8 +293,8 @@ def run_fbp(args):
         stage_ms, ncalls = h.profile_read(1)
         h.profile(False)
         rate = ns * args.steps / dt
-        L = 4 * g["Nrho"] * g["Ntheta"]
-        b_alg = 4 * N0 * P0 + 16 * g["N"] * g["Nproj"] + 12 * L + 4 * N0 * N0
+        from lprec_amd import traffic
+        b_alg = traffic.b_adj(g, filtered=True)
         # reconstruction error against the phantom inside the disk (scale-free)
         x = rec.cpu().numpy().astype(np.float64)
         a = float(np.sum(x * f_true) / np.sum(x * x))
@@ -409,7 +408,8 @@ def run_multiscan(args):
     slices each: fwd+adj of every scan per step, one graph launch per step."""
     import numpy as np
     import torch
-    from lprec_amd.batch import MultiScan, bytes_per_slice, random_scans
+    from lprec_amd.batch import MultiScan, random_scans
+    from lprec_amd.traffic import bytes_per_slice
     from lprec_amd.phantom import phantom_stack
     scans = random_scans(96, seed=96)
     t0 = time.perf_counter()
@@ -422,7 +422,7 @@ def run_multiscan(args):
         tuned = {k: v[0] for k, v in ms.autotune(reps=2, sweeps=1).items()}
     ms.capture()
     nsl = ms.nslices
-    b_alg = sum(bytes_per_slice(s.N0, s.Nproj) * s.nslices for s in scans)
+    b_alg = sum(bytes_per_slice(s.N0, s.Nproj, s.cor) * s.nslices for s in scans)
     t_graph = _timed(ms.replay, args.steps, args.warmup)
     t_conc = _timed(lambda: ms.step_eager(True), args.steps, args.warmup)
     t_seq = _timed(lambda: ms.step_eager(False), args.steps, args.warmup)

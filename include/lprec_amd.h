@@ -117,7 +117,13 @@ int lp_info(const lp_handle* h, int64_t* info);
  * before lp_init_fwd / lp_init_adj), key 6 = forward operator's inverse row
  * pass and polar gather fused into one kernel (1) or run as two (0, the
  * default); the fused kernel applies with 16 rows per row-FFT block and the
- * radix-16 kernels.  Returns LP_ERR_ARG for bad values. */
+ * radix-16 kernels, key 7 = lp_fwd_T's transpose of the polar gather as two
+ * atomics-free gathers (1, the default, where the geometry allows it; reads
+ * back 0 where it does not) or as the tiled scatter (0), key 8 = the tiled
+ * scatters of lp_fwd_T / lp_adj_T flush their source boxes through a staging
+ * area summed per destination tile (1, the default; no atomics into the
+ * destination) or with global atomics (0).  Returns LP_ERR_ARG for bad
+ * values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 

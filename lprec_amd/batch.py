@@ -14,6 +14,7 @@ import torch
 
 from .lpTransform import lpTransform
 from . import tune as tuner
+from .traffic import bytes_per_slice  # B_alg of one fwd+adj slice (shared with bench.py)
 
 
 class Scan:
@@ -136,15 +137,6 @@ def random_scans(n, seed, sizes=(512, 1024, 2048), angles=(360, 720, 1500), slic
     rng = np.random.default_rng(seed)
     return [Scan(int(rng.choice(sizes)), int(rng.choice(angles)), int(rng.integers(slices[0], slices[1] + 1)))
             for _ in range(n)]
-
-
-def bytes_per_slice(N0, P0, N=None):
-    """Algorithmic HBM bytes of one fwd+adj slice (the B_alg model)."""
-    N = N0 if N is None else N
-    os_ = max(int(round(3 * N / (2 * P0))), 1)
-    P = os_ * P0
-    L = 4 * (2 * N) * N
-    return (4 * N0 * N0 + 16 * N * N + 12 * L + 4 * N0 * P0) + (4 * N0 * P0 + 16 * N * P + 12 * L + 4 * N0 * N0)
 
 
 __all__ = ["Scan", "MultiScan", "random_scans", "bytes_per_slice"]

@@ -6,11 +6,14 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fft64.hpp"
@@ -65,6 +68,7 @@ struct lp_handle {
     // where the separate pass reads one, and it measured no faster than the
     // separate pair (1.91 vs 1.90 ms per 32 slices at N = 2048)
     int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1, fuse_polar = 0;
+    int polar_t = 1;   // separable polar transpose where the geometry allows (else the tiled scatter)
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -87,6 +91,29 @@ struct lp_handle {
     int2* d_rangeA = nullptr;
     int2* d_wrangeF = nullptr;   // theta columns of G each (span, row) the polar gather reads
     int2* d_wrangeA = nullptr;   // ... and the Cartesian gather reads
+    // staged (atomics-free) flush of the tiled scatters of the transposes:
+    // per tile list, each box's offset and rectangle and, per destination
+    // tile of 32 x 32, the boxes that cover it
+    struct Stage {
+        int* off = nullptr;
+        int4* rect = nullptr;
+        int* tbeg = nullptr;
+        int* tbox = nullptr;
+        long long total = 0;   // floats per slice
+        int tiles_x = 0, tiles_y = 0, nplanes = 1;
+        bool ok = false;
+    } stF, stA, stC;
+    int staged = 1;   // use the staged flush where the half-spectrum buffer holds it
+    // colour-sorted points of the tile lists (atomics-free scatters)
+    unsigned short *d_permF = nullptr, *d_permP = nullptr, *d_permA = nullptr, *d_permC = nullptr;
+    int *d_clsF = nullptr, *d_clsP = nullptr, *d_clsA = nullptr, *d_clsC = nullptr;
+    // separable transpose of the polar gather (pt_ok false: the geometry does not allow it)
+    int* d_pt_abeg = nullptr;
+    float* d_pt_cxa = nullptr;
+    int* d_pt_xs = nullptr;
+    int* d_pt_aj = nullptr;
+    int pt_nA = 0, pt_qlo = 0, pt_Q = 0, pt_jlo = 0, pt_jn = 0;
+    bool pt_ok = false;
     // the fused inverse row pass + polar gather: samples per (span, band of 13 base rows)
     float2* d_pcoord = nullptr;
     int* d_poff = nullptr;
@@ -239,6 +266,8 @@ void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>
     for (size_t i = 0; i < spans.size(); ++i) {
         lpk::GTile& g = t[i];
         g.r0 = r0; g.c0 = c0; g.nr = nr; g.nc = nc; g.span = spans[i]; g.lnc = __builtin_ctz((unsigned)nc);
+        g.cr = g.cc = 0;
+        g.pbase = g.cbase = 0;
         tile_box(coord, spans[i], r0, c0, nr, nc, g.x0, g.y0, g.bw, g.bh);
         if ((long long)(g.bw | 1) * (g.bh | 1) > cap) fits = false;   // either box layout fits
     }
@@ -259,6 +288,106 @@ void add_tiles(std::vector<lpk::GTile>& out, Coord coord, const std::vector<int>
         maxbox = std::max(maxbox, (g.bw | 1) * (g.bh | 1));
         out.push_back(g);
     }
+}
+
+// Colour classes of a tile for the transposed (scatter) use: the smallest
+// (cr, cc) found such that points (r, c) and (r', c') with r = r' (mod cr)
+// and c = c' (mod cc) never share a cubic tap (their floor coordinates differ
+// by 4 or more in x or in y).  Nearest clashes along each axis set the
+// periods; same-colour neighbours up to two periods away are then checked
+// and a period is widened until none clash.  More than 256 classes (or none
+// found): cr = 0, the scatter accumulates with atomics.
+template <class Coord>
+void colour_tile(lpk::GTile& t, Coord coord) {
+    const int nr = t.nr, nc = t.nc;
+    std::vector<int> fx((size_t)nr * nc), fy((size_t)nr * nc);
+    for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nc; ++c) {
+            const float2 v = coord(t.span, t.r0 + r, t.c0 + c);
+            const bool ok = valid2(v);
+            fx[(size_t)r * nc + c] = ok ? (int)std::floor(v.x) : INT_MIN;
+            fy[(size_t)r * nc + c] = ok ? (int)std::floor(v.y) : 0;
+        }
+    auto clash = [&](int r1, int c1, int r2, int c2) {
+        if (r2 < 0 || r2 >= nr || c2 < 0 || c2 >= nc) return false;
+        const size_t i = (size_t)r1 * nc + c1, j = (size_t)r2 * nc + c2;
+        if (fx[i] == INT_MIN || fx[j] == INT_MIN) return false;
+        return std::abs(fx[i] - fx[j]) <= 3 && std::abs(fy[i] - fy[j]) <= 3;
+    };
+    int cr = 1, cc = 1;
+    for (int r = 0; r < nr; ++r)
+        for (int c = 0; c < nc; ++c) {
+            for (int d = 1; d < std::min(nr, 64); ++d)
+                if (clash(r, c, r + d, c)) cr = std::max(cr, d + 1);
+            for (int d = 1; d < std::min(nc, 64); ++d)
+                if (clash(r, c, r, c + d)) cc = std::max(cc, d + 1);
+        }
+    for (int iter = 0; iter < 64 && cr * cc <= 256; ++iter) {
+        bool bad = false;
+        for (int r = 0; r < nr && !bad; ++r)
+            for (int c = 0; c < nc && !bad; ++c)
+                for (int a = -2; a <= 2 && !bad; ++a)
+                    for (int b = -2; b <= 2 && !bad; ++b)
+                        if ((a || b) && clash(r, c, r + a * cr, c + b * cc)) {
+                            bad = true;
+                            if (cr <= cc) ++cr; else ++cc;
+                        }
+        if (!bad) {
+            t.cr = std::min(cr, nr);
+            t.cc = std::min(cc, nc);
+            return;
+        }
+    }
+    t.cr = t.cc = 0;
+}
+
+// Colours every tile of a list (threads over tiles) and lays out the
+// colour-sorted point lists: perm (r * nc + c of the points with a sample,
+// stable within a class) and the class bounds cls, both concatenated over the
+// tiles (GTile.pbase / cbase); uploaded to *d_perm / *d_cls.
+template <class Coord>
+int colour_list(lp_handle* h, std::vector<lpk::GTile>& tiles, Coord coord, unsigned short** d_perm, int** d_cls) {
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<unsigned short>> perms(tiles.size());
+    std::vector<std::vector<int>> clss(tiles.size());
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nth; ++w)
+        pool.emplace_back([&, w] {
+            for (size_t i = w; i < tiles.size(); i += nth) {
+                lpk::GTile& t = tiles[i];
+                colour_tile(t, coord);
+                if (t.cr <= 0) continue;
+                const int ncol = t.cr * t.cc;
+                std::vector<int> cnt(ncol + 1, 0);
+                std::vector<int> col((size_t)t.nr * t.nc, -1);
+                for (int r = 0; r < t.nr; ++r)
+                    for (int c = 0; c < t.nc; ++c)
+                        if (valid2(coord(t.span, t.r0 + r, t.c0 + c))) {
+                            const int k = (r % t.cr) * t.cc + c % t.cc;
+                            col[(size_t)r * t.nc + c] = k;
+                            cnt[k + 1]++;
+                        }
+                for (int k = 0; k < ncol; ++k) cnt[k + 1] += cnt[k];
+                std::vector<unsigned short> pm(cnt[ncol]);
+                std::vector<int> at(cnt.begin(), cnt.end() - 1);
+                for (size_t p = 0; p < col.size(); ++p)
+                    if (col[p] >= 0) pm[at[col[p]]++] = (unsigned short)p;
+                perms[i] = std::move(pm);
+                clss[i] = std::move(cnt);
+            }
+        });
+    for (auto& th : pool) th.join();
+    std::vector<unsigned short> perm;
+    std::vector<int> cls;
+    for (size_t i = 0; i < tiles.size(); ++i) {
+        tiles[i].pbase = (int)perm.size();
+        tiles[i].cbase = (int)cls.size();
+        perm.insert(perm.end(), perms[i].begin(), perms[i].end());
+        cls.insert(cls.end(), clss[i].begin(), clss[i].end());
+    }
+    int rc;
+    if ((rc = h->upload(d_perm, perm)) || (rc = h->upload(d_cls, cls))) return rc;
+    return LP_OK;
 }
 
 template <class Coord, class Keep>
@@ -410,6 +539,166 @@ int scratch_address(int device, void** out) {
 }
 
 inline size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// LPREC_AMD_VERBOSE=1: one line per tile list on stderr at init (tiles,
+// coloured tiles and their mean / largest class count, staged floats per slice).
+void report_tiles(const char* name, const std::vector<lpk::GTile>& tiles, long long staged) {
+    const char* v = std::getenv("LPREC_AMD_VERBOSE");
+    if (!v || v[0] != '1') return;
+    size_t coloured = 0;
+    long long sum = 0, mx = 0;
+    for (const auto& t : tiles)
+        if (t.cr > 0) {
+            ++coloured;
+            sum += (long long)t.cr * t.cc;
+            mx = std::max(mx, (long long)t.cr * t.cc);
+        }
+    std::fprintf(stderr, "lprec_amd: %s: %zu tiles, %zu coloured (mean %.1f, max %lld classes), %lld staged floats/slice\n",
+                 name, tiles.size(), coloured, coloured ? (double)sum / coloured : 0.0, mx, staged);
+}
+
+// Staging tables of a tile list (launch_scatter with Scatter.stage, then
+// launch_box_sum): box i = tiles[i] (tile * group + span slot), its offset in
+// a slice's staging area, and the destination tiles (32 x 32 points of a
+// W x H plane; plane = the box's span when per_span) that it covers with
+// periodic placement, listed per destination tile in increasing box order.
+int stage_tables(lp_handle* h, const std::vector<lpk::GTile>& tiles, int W, int H, bool per_span,
+                 lp_handle::Stage& st) {
+    const int tx = (W + 31) / 32, ty = (H + 31) / 32, np = per_span ? 3 : 1;
+    std::vector<int> off(tiles.size());
+    std::vector<int4> rect(tiles.size());
+    std::vector<std::vector<int>> lists((size_t)np * tx * ty);
+    long long total = 0;
+    for (size_t i = 0; i < tiles.size(); ++i) {
+        const lpk::GTile& t = tiles[i];
+        off[i] = (int)total;
+        rect[i] = make_int4(t.x0, t.y0, std::max(t.bw, 0), std::max(t.bh, 0));
+        if (t.bw <= 0 || t.bh <= 0) continue;
+        // a box wider than the plane, or an origin beyond one period: keep the atomic flush
+        if (t.bw > W || t.bh > H || t.x0 <= -W || t.x0 >= W || t.y0 <= -H || t.y0 >= H) return LP_OK;
+        total += (long long)t.bw * t.bh;
+        if (total >= (1LL << 31)) return LP_OK;
+        const int plane = per_span ? t.span : 0;
+        // destination tile columns / rows the wrapped box touches
+        auto cover = [](int o, int n, int L, int tile, std::vector<int>& out) {
+            std::vector<char> hit((L + tile - 1) / tile, 0);
+            for (int d = 0; d < n; ++d) hit[(((o + d) % L) + L) % L / tile] = 1;
+            out.clear();
+            for (int k = 0; k < (int)hit.size(); ++k)
+                if (hit[k]) out.push_back(k);
+        };
+        std::vector<int> cx, cy;
+        cover(t.x0, t.bw, W, 32, cx);
+        cover(t.y0, t.bh, H, 32, cy);
+        for (int yy : cy)
+            for (int xx : cx) lists[((size_t)plane * ty + yy) * tx + xx].push_back((int)i);
+    }
+    std::vector<int> tbeg(lists.size() + 1, 0), tbox;
+    for (size_t k = 0; k < lists.size(); ++k) {
+        tbox.insert(tbox.end(), lists[k].begin(), lists[k].end());
+        tbeg[k + 1] = (int)tbox.size();
+    }
+    int rc;
+    if ((rc = h->upload(&st.off, off)) || (rc = h->upload(&st.rect, rect)) || (rc = h->upload(&st.tbeg, tbeg)) ||
+        (rc = h->upload(&st.tbox, tbox)))
+        return rc;
+    st.total = total;
+    st.tiles_x = tx;
+    st.tiles_y = ty;
+    st.nplanes = np;
+    st.ok = true;
+    return LP_OK;
+}
+
+// Tables of the separable polar transpose (launch_polar_transpose): the
+// angle range of each span, each kept angle's theta coordinate, the first
+// detector cell per angle and unwrapped rho row, the first angle per span and
+// theta column.  Leaves pt_ok false when the polar map is not separable in
+// the way the kernels need (a theta coordinate that varies along the
+// detector, rho not monotone, an angle whose taps wrap around rho more than
+// once, taps that wrap around the theta axis).
+int polar_transpose_tables(lp_handle* h, const std::vector<float2>& p2lp, const std::vector<unsigned char>& spanrow) {
+    const int N = h->N, Nr = h->Nr, Nt = h->Nt, P0 = h->P0, N0 = h->N0, os = h->os;
+    const int lb = N / 2 - h->cor;
+    auto c_of = [&](int a, int x) { return p2lp[(size_t)a * os * N + x + lb]; };
+    std::vector<int> abeg(6, 0);
+    for (int k = 0; k < 3; ++k) {
+        int ab = -1, ae = -1;
+        for (int a = 0; a < P0; ++a)
+            if (spanrow[(size_t)a * os] == k) { if (ab < 0) ab = a; ae = a + 1; }
+        if (ab < 0) ab = ae = 0;
+        abeg[k] = ab;
+        abeg[3 + k] = ae;
+    }
+    std::vector<float> cxa(P0);
+    long long qmin = LLONG_MAX, qmax = LLONG_MIN;
+    for (int a = 0; a < P0; ++a) {
+        cxa[a] = c_of(a, 0).x;
+        long long prev = LLONG_MIN;
+        for (int x = 0; x < N0; ++x) {
+            const float2 c = c_of(a, x);
+            if (c.x != cxa[a]) return LP_OK;
+            const long long iy = (long long)std::floor(c.y);
+            if (iy < prev) return LP_OK;
+            prev = iy;
+        }
+        // an angle's taps may wrap around rho once (the span edges reach
+        // across the whole grid): at most two unwrapped rows per grid row
+        const long long i0 = (long long)std::floor(c_of(a, 0).y), i1 = prev;
+        if (i1 - i0 + 4 >= 2LL * Nr) return LP_OK;
+        qmin = std::min(qmin, i0);
+        qmax = std::max(qmax, i1);
+    }
+    const long long qlo = qmin - 3, Q = qmax + 3 - qlo;
+    if (Q > 4LL * Nr) return LP_OK;
+    std::vector<int> xs((size_t)P0 * (Q + 1));
+    for (int a = 0; a < P0; ++a) {
+        int x = 0;
+        for (long long i = 0; i <= Q; ++i) {
+            while (x < N0 && (long long)std::floor(c_of(a, x).y) < qlo + i) ++x;
+            xs[(size_t)a * (Q + 1) + i] = x;
+        }
+    }
+    long long imin = LLONG_MAX, imax = LLONG_MIN;
+    for (int k = 0; k < 3; ++k)
+        for (int a = abeg[k]; a < abeg[3 + k]; ++a) {
+            const long long ix = (long long)std::floor(cxa[a]);
+            if (a > abeg[k] && ix < (long long)std::floor(cxa[a - 1])) return LP_OK;
+            imin = std::min(imin, ix);
+            imax = std::max(imax, ix);
+        }
+    if (imin > imax) return LP_OK;
+    for (int k = 0; k < 3; ++k)   // the intermediate [3][Nr][angles] must fit the half-spectrum buffer
+        if (abeg[3 + k] - abeg[k] > Nt) return LP_OK;
+    // the cubic tap extent also for linear sampling (zeros in its outer
+    // columns): it is the extent of the gather boxes, the columns the row pass
+    // then reads, so every one of them must be written
+    const long long jlo = imin - 1, jhi = imax + 3;
+    if (jlo < 0 || jhi > Nt) return LP_OK;
+    const int jn = (int)(jhi - jlo);
+    std::vector<int> aj(3 * (size_t)(jn + 5));
+    int nA = 1;
+    for (int k = 0; k < 3; ++k) {
+        const int na = abeg[3 + k] - abeg[k];
+        nA = std::max(nA, na);
+        int al = 0;
+        for (int i = 0; i <= jn + 4; ++i) {
+            while (al < na && (long long)std::floor(cxa[abeg[k] + al]) < jlo - 2 + i) ++al;
+            aj[(size_t)k * (jn + 5) + i] = al;
+        }
+    }
+    int rc;
+    if ((rc = h->upload(&h->d_pt_abeg, abeg)) || (rc = h->upload(&h->d_pt_cxa, cxa)) ||
+        (rc = h->upload(&h->d_pt_xs, xs)) || (rc = h->upload(&h->d_pt_aj, aj)))
+        return rc;
+    h->pt_nA = nA;
+    h->pt_qlo = (int)qlo;
+    h->pt_Q = (int)Q;
+    h->pt_jlo = (int)jlo;
+    h->pt_jn = jn;
+    h->pt_ok = true;
+    return LP_OK;
+}
 
 }  // namespace
 
@@ -568,6 +857,14 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
     h->ntilesF = (int)tF.size();
     h->ntilesP = (int)tP.size();
     std::vector<int2> wrng = box_row_ranges(tP, Nr, Nt);
+    {   // colour classes of the tiles for the transposed (scatter) use
+        const int lbp = N / 2 - h->cor, osp = h->os;
+        auto coordF = [&](int k, int r, int c) { return cmap[(size_t)k * npts + (size_t)r * Nt + c]; };
+        auto coordP = [&](int, int a, int x) { return p2lp[(size_t)a * osp * N + x + lbp]; };
+        if ((rc = colour_list(h, tF, coordF, &h->d_permF, &h->d_clsF)) ||
+            (rc = colour_list(h, tP, coordP, &h->d_permP, &h->d_clsP)))
+            return rc;
+    }
     if ((rc = h->upload(&h->d_tilesF, tF)) || (rc = h->upload(&h->d_tilesP, tP))) return rc;
     if ((rc = h->upload(&h->d_wrangeF, wrng))) return rc;
     {
@@ -580,6 +877,10 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
         (rc = h->upload(&h->d_spanrow, spanrow)) || (rc = h->upload(&h->d_fzF, fzT)) ||
         (rc = h->alloc(&h->d_fc, (size_t)h->Ns * N * N)))
         return rc;
+    if ((rc = polar_transpose_tables(h, p2lp, spanrow))) return rc;
+    if ((rc = stage_tables(h, tF, N, N, false, h->stF))) return rc;
+    report_tiles("image->log-polar", tF, h->stF.total);
+    report_tiles("log-polar->polar", tP, 0);
     {
         // polar samples of the kept angles by (span, band of kFuseBase base
         // rows): their table coordinates and output offsets a * N0 + x
@@ -715,6 +1016,14 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
     h->ntilesA = (int)tA.size();
     h->ntilesC = (int)tC.size();
     std::vector<int2> wrng = box_row_ranges(tC, Nr, Nt);
+    {   // colour classes of the tiles for the transposed (scatter) use
+        const int o = N / 2 - h->N0 / 2;
+        auto coordA = [&](int k, int r, int c) { return cmap[(size_t)k * npts + (size_t)r * Nt + c]; };
+        auto coordC = [&](int k, int y, int x) { return cmapC[(size_t)k * NN + (size_t)(y + o) * N + x + o]; };
+        if ((rc = colour_list(h, tA, coordA, &h->d_permA, &h->d_clsA)) ||
+            (rc = colour_list(h, tC, coordC, &h->d_permC, &h->d_clsC)))
+            return rc;
+    }
     if ((rc = h->upload(&h->d_wrangeA, wrng))) return rc;
     {
         std::vector<int2> in_rows = data_rows(rng, Nr);
@@ -723,6 +1032,10 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         if (!out_rows.empty() && (rc = h->upload(&h->d_outrowsA, out_rows))) return rc;
     }
     if ((rc = h->upload(&h->d_tilesA, tA)) || (rc = h->upload(&h->d_tilesC, tC))) return rc;
+    if ((rc = stage_tables(h, tA, N, h->P, false, h->stA)) || (rc = stage_tables(h, tC, Nt, Nr, true, h->stC)))
+        return rc;
+    report_tiles("sinogram->log-polar", tA, h->stA.total);
+    report_tiles("log-polar->Cartesian", tC, h->stC.total);
     if ((rc = h->upload(&h->d_cmapA, cmap)) || (rc = h->upload(&h->d_rangeA, rng)) || (rc = h->upload(&h->d_cmapC, cmapC)) ||
         (rc = h->upload(&h->d_fzA, fzT)) || (rc = h->alloc(&h->d_Rc, (size_t)h->Ns * h->P * N)))
         return rc;
@@ -758,6 +1071,42 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
     }
     h->has_adj = true;
     return LP_OK;
+}
+
+// Launch a tiled scatter; with the staged flush (when enabled and built) the
+// boxes go to the half-spectrum buffer -- free during every scatter of the
+// transposes -- in chunks of slices that fit it, and launch_box_sum forms the
+// destination, which needs no clearing; otherwise the destination is cleared
+// and the boxes are added with atomics.
+static void scatter_to(lp_handle* h, lpk::Scatter g, const lpk::GTile* tiles, int ntiles, int kind,
+                       const lp_handle::Stage& st, size_t dst_count, hipStream_t s) {
+    const size_t cap = 3 * (size_t)h->Ns * h->h * h->Nr * 2;   // floats of S
+    const int chunk = st.ok && st.total > 0 ? (int)std::min<size_t>(cap / (size_t)st.total, (size_t)g.nz) : 0;
+    if (h->staged && chunk >= 1) {
+        // slices in chunks whose boxes fit the staging area
+        const int nz = g.nz;
+        for (int z0 = 0; z0 < nz; z0 += chunk) {
+            lpk::Scatter c = g;
+            c.nz = std::min(chunk, nz - z0);
+            c.val = g.val + (size_t)z0 * g.v_z;
+            c.dst = g.dst + (size_t)z0 * g.dst_z;
+            c.stage = reinterpret_cast<float*>(h->d_S);
+            c.stage_z = st.total;
+            c.box_off = st.off;
+            lpk::launch_scatter(c, tiles, ntiles, kind, s);
+            lpk::BoxSum b{};
+            b.stage = c.stage; b.stage_z = st.total;
+            b.tile_beg = st.tbeg; b.tile_box = st.tbox; b.box_rect = st.rect; b.box_off = st.off;
+            b.dst = c.dst; b.dst_z = g.dst_z; b.dst_plane = g.dst_span;
+            b.W = g.W; b.H = g.H; b.tiles_x = st.tiles_x; b.tiles_y = st.tiles_y; b.nplanes = st.nplanes;
+            b.nz = c.nz;
+            lpk::launch_box_sum(b, s);
+        }
+        return;
+    }
+    lpk::launch_zero(g.dst, dst_count, s);
+    g.stage = nullptr;
+    lpk::launch_scatter(g, tiles, ntiles, kind, s);
 }
 
 static lpk::RowPass row_pass(lp_handle* h, int nz) {
@@ -970,24 +1319,38 @@ int lp_fwd_T(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     (void)hipGetLastError();
     const int N = h->N;
     const long long grid = (long long)h->Nr * h->Nt;
-    lpk::launch_zero(h->d_G, 3 * (size_t)ns0 * grid, s);
-    {   // transpose of the log-polar -> polar gather
+    const int2* g_range = nullptr;   // theta columns of the transposed grid that can be non-zero
+    if (h->pt_ok && h->polar_t) {
+        // transpose of the log-polar -> polar gather as two gathers; the
+        // intermediate lives in the half-spectrum buffer, free until the row pass
+        lpk::PolarT p{};
+        p.val = R; p.v_z = (long long)h->P0 * h->N0;
+        p.coord = h->d_p2lp; p.c_row = h->os * N; p.c_co = N / 2 - h->cor;
+        p.cxa = h->d_pt_cxa; p.u = reinterpret_cast<float*>(h->d_S); p.nA = h->pt_nA; p.abeg = h->d_pt_abeg;
+        p.xs = h->d_pt_xs; p.qlo = h->pt_qlo; p.Q = h->pt_Q;
+        p.G = h->d_G; p.aj = h->d_pt_aj; p.jlo = h->pt_jlo; p.jn = h->pt_jn;
+        p.Nr = h->Nr; p.Nt = h->Nt; p.N0 = h->N0; p.nz = ns0; p.cubic = h->cubic;
+        lpk::launch_polar_transpose(p, s);
+        g_range = h->d_wrangeF;   // the polar taps' columns; zero elsewhere (not written)
+    } else {
+        lpk::launch_zero(h->d_G, 3 * (size_t)ns0 * grid, s);
+        // transpose of the log-polar -> polar gather
         lpk::Scatter g{};
         g.dst = h->d_G; g.dst_z = grid; g.dst_span = (long long)ns0 * grid; g.W = h->Nt; g.H = h->Nr;
         g.coord = h->d_p2lp; g.c_span = 0; g.c_row = h->os * N; g.c_ro = 0; g.c_co = N / 2 - h->cor;
         g.scale = nullptr; g.s_row = 0;
         g.val = R; g.v_z = (long long)h->P0 * h->N0; g.v_span = 0; g.v_row = h->N0;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxP;
+        g.perm = h->d_permP; g.cls = h->d_clsP;
         lpk::launch_scatter(g, h->d_tilesP, h->ntilesP, 2, s);
     }
     lpk::RowPass rp = row_pass(h, ns0);
     rp.conj = 1;
     lpk::Src src{h->d_fc, N, N, (long long)N * N};
-    lpk::launch_rows_fwd(rp, h->d_G, nullptr, s);
+    lpk::launch_rows_fwd(rp, h->d_G, g_range, s);
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzF, s);
     lpk::launch_rows_inv(rp, h->d_G, h->d_rangeF, s);   // the scatter reads the sampled columns only
     float* acc = h->d_fc;   // never aliases G, which the scatter reads
-    lpk::launch_zero(acc, (size_t)ns0 * N * N, s);
     {   // transpose of the image -> log-polar gather (x e^rho)
         lpk::Scatter g{};
         g.dst = acc; g.dst_z = (long long)N * N; g.dst_span = 0; g.W = N; g.H = N;
@@ -995,7 +1358,8 @@ int lp_fwd_T(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
         g.scale = h->d_erho; g.s_row = h->Nt;
         g.val = h->d_G; g.v_z = grid; g.v_span = (long long)ns0 * grid; g.v_row = h->Nt;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxF;
-        lpk::launch_scatter(g, h->d_tilesF, h->ntilesF, 0, s);
+        g.perm = h->d_permF; g.cls = h->d_clsF;
+        scatter_to(h, g, h->d_tilesF, h->ntilesF, 0, h->stF, (size_t)ns0 * N * N, s);
     }
     if (h->cubic) {
         lpk::launch_col_iir_T(h->d_fc, h->d_tmp, ns0, N, N, h->colchunk, s);
@@ -1017,7 +1381,6 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     (void)hipGetLastError();
     const int N = h->N;
     const long long grid = (long long)h->Nr * h->Nt;
-    lpk::launch_zero(h->d_G, 3 * (size_t)ns0 * grid, s);
     {   // transpose of the log-polar -> Cartesian gather (three spans per tile)
         const int o = N / 2 - h->N0 / 2;
         lpk::Scatter g{};
@@ -1026,7 +1389,8 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
         g.scale = nullptr; g.s_row = 0;
         g.val = f; g.v_z = (long long)h->N0 * h->N0; g.v_span = 0; g.v_row = h->N0;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 3; g.cubic = h->cubic; g.box_floats = h->boxC;
-        lpk::launch_scatter(g, h->d_tilesC, h->ntilesC, 3, s);
+        g.perm = h->d_permC; g.cls = h->d_clsC;
+        scatter_to(h, g, h->d_tilesC, h->ntilesC, 3, h->stC, 3 * (size_t)ns0 * grid, s);
     }
     lpk::RowPass rp = row_pass(h, ns0);
     rp.conj = 1;
@@ -1035,7 +1399,6 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     lpk::launch_cols(rp, h->plan_M, h->tw[h->Nr], h->d_fzA, s);
     lpk::launch_rows_inv(rp, h->d_G, h->d_rangeA, s);
     float* acc = h->d_Rc;
-    lpk::launch_zero(acc, (size_t)ns0 * h->P * N, s);
     {   // transpose of the sinogram -> log-polar gather, plus its extra samples
         lpk::Scatter g{};
         g.dst = acc; g.dst_z = (long long)h->P * N; g.dst_span = 0; g.W = N; g.H = h->P;
@@ -1043,7 +1406,8 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
         g.scale = nullptr; g.s_row = 0;
         g.val = h->d_G; g.v_z = grid; g.v_span = (long long)ns0 * grid; g.v_row = h->Nt;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxA;
-        lpk::launch_scatter(g, h->d_tilesA, h->ntilesA, 1, s);
+        g.perm = h->d_permA; g.cls = h->d_clsA;
+        scatter_to(h, g, h->d_tilesA, h->ntilesA, 1, h->stA, (size_t)ns0 * h->P * N, s);
         lpk::launch_scatter_extra(h->d_G, acc, ns0, h->Nt, h->Nr, N, h->P, h->d_xrow, h->d_xent, h->cubic, s);
     }
     lpk::SinoPrep sp;
@@ -1129,6 +1493,14 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "polar fusion must be 0 or 1");
             h->fuse_polar = value;
             return LP_OK;
+        case 7:
+            if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "separable polar transpose must be 0 or 1");
+            h->polar_t = value;
+            return LP_OK;
+        case 8:
+            if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "staged scatter flush must be 0 or 1");
+            h->staged = value;
+            return LP_OK;
         case 5:
             if (h->has_fwd || h->has_adj)
                 return h->fail(LP_ERR_ARG, "the gather box budget is fixed once lp_init_fwd / lp_init_adj ran");
@@ -1204,6 +1576,8 @@ int lp_get_tuning(const lp_handle* h, int key, int* value) {
         case 4: *value = h->fft16; return LP_OK;
         case 5: *value = h->box_cap; return LP_OK;
         case 6: *value = h->fuse_polar; return LP_OK;
+        case 7: *value = h->pt_ok ? h->polar_t : 0; return LP_OK;
+        case 8: *value = h->staged; return LP_OK;
     }
     return LP_ERR_ARG;
 }

@@ -1978,6 +1978,11 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
     const GTile t0 = tb[0];
     const int npts = t0.nr * t0.nc;
     const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    if (g.perm) {   // k_scatter_col handles the coloured tiles
+        bool any = false;
+        for (int gi = 0; gi < g.group; ++gi) any |= tb[gi].bw > 0 && tb[gi].cr == 0;
+        if (!any) return;
+    }
     // Loads are unconditional at clamped point indices (a guarded load
     // compiles to a branch and a wait of its own), all PPT in flight at once.
     const float inv_nc0 = 1.0f / (float)t0.nc;
@@ -2003,7 +2008,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
             if (threadIdx.x + e * nt >= npts) val[e] = 0.f;
         for (int gi = 0; gi < g.group; ++gi) {
             const GTile t = tb[gi];
-            if (t.bw <= 0) continue;   // uniform over the block
+            if (t.bw <= 0 || (g.perm && t.cr > 0)) continue;   // uniform over the block
             const int bs = t.bw | 1;
             const int nbox = bs * t.bh;
             const float2* cbase = g.coord + (size_t)t.span * g.c_span;
@@ -2016,48 +2021,63 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
             }
             for (int i = threadIdx.x; i < nbox; i += nt) box[i] = 0.f;
             __syncthreads();
+            // one point's 16 (4) weighted taps into the box, atomically
+            auto deposit = [&](int e, bool atomic) {
+                const float2 c = cc[e];
+                float fx = floorf(c.x), fy = floorf(c.y);
+                float ax = c.x - fx, ay = c.y - fy;
+                int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
+                int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
+                float* q = box + ly * bs + lx;
+                const float v = val[e];
+                auto add = [&](float* a, float w) {
+                    if (atomic) atomicAdd(a, w);
+                    else *a += w;
+                };
+                if (g.cubic) {
+                    float wx[4], wy[4];
+                    bspline_w(ax, wx);
+                    bspline_w(ay, wy);
 #pragma unroll
-            for (int e = 0; e < PPT; ++e) {
-                int p = threadIdx.x + e * nt;
-                if (p < npts && val[e] != 0.f) {
-                    const float2 c = cc[e];
-                    if (c.x == c.x) {
-                        float fx = floorf(c.x), fy = floorf(c.y);
-                        float ax = c.x - fx, ay = c.y - fy;
-                        int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
-                        int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
-                        float* q = box + ly * bs + lx;
-                        const float v = val[e];
-                        if (g.cubic) {
-                            float wx[4], wy[4];
-                            bspline_w(ax, wx);
-                            bspline_w(ay, wy);
+                    for (int b = 0; b < 4; ++b) {
+                        float vb = wy[b] * v;
 #pragma unroll
-                            for (int b = 0; b < 4; ++b) {
-                                float vb = wy[b] * v;
-#pragma unroll
-                                for (int a2 = 0; a2 < 4; ++a2) atomicAdd(q + b * bs + a2, wx[a2] * vb);
-                            }
-                        } else {
-                            float* q1 = q + bs + 1;
-                            float* q2 = q1 + bs;
-                            atomicAdd(q1, (1.f - ay) * (1.f - ax) * v);
-                            atomicAdd(q1 + 1, (1.f - ay) * ax * v);
-                            atomicAdd(q2, ay * (1.f - ax) * v);
-                            atomicAdd(q2 + 1, ay * ax * v);
-                        }
+                        for (int a2 = 0; a2 < 4; ++a2) add(q + b * bs + a2, wx[a2] * vb);
                     }
+                } else {
+                    float* q1 = q + bs + 1;
+                    float* q2 = q1 + bs;
+                    add(q1, (1.f - ay) * (1.f - ax) * v);
+                    add(q1 + 1, (1.f - ay) * ax * v);
+                    add(q2, ay * (1.f - ax) * v);
+                    add(q2 + 1, ay * ax * v);
                 }
+            };
+            {
+#pragma unroll
+                for (int e = 0; e < PPT; ++e) {
+                    int p = threadIdx.x + e * nt;
+                    if (p < npts && val[e] != 0.f && cc[e].x == cc[e].x) deposit(e, true);
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            float* D = g.dst + (size_t)z * g.dst_z + (size_t)t.span * g.dst_span;
             const float inv = 1.0f / (float)t.bw;
             const int nb = t.bw * t.bh;
-            for (int i = threadIdx.x; i < nb; i += nt) {
-                int yy = (int)(((float)i + 0.5f) * inv);
-                int xx = i - yy * t.bw;
-                float v = box[yy * bs + xx];
-                if (v != 0.f) unsafeAtomicAdd(D + (size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W), v);
+            if (g.stage) {
+                float* B = g.stage + (size_t)z * g.stage_z + g.box_off[(size_t)tile * g.group + gi];
+                for (int i = threadIdx.x; i < nb; i += nt) {
+                    int yy = (int)(((float)i + 0.5f) * inv);
+                    int xx = i - yy * t.bw;
+                    B[i] = box[yy * bs + xx];
+                }
+            } else {
+                float* D = g.dst + (size_t)z * g.dst_z + (size_t)t.span * g.dst_span;
+                for (int i = threadIdx.x; i < nb; i += nt) {
+                    int yy = (int)(((float)i + 0.5f) * inv);
+                    int xx = i - yy * t.bw;
+                    float v = box[yy * bs + xx];
+                    if (v != 0.f) unsafeAtomicAdd(D + (size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W), v);
+                }
             }
             __syncthreads();
         }
@@ -2104,6 +2124,94 @@ void launch_zero(float* p, size_t n, hipStream_t s) {
     if (n % 4) hipLaunchKernelGGL(k_zero_tail, dim3(1), dim3(4), 0, s, p, n4 * 4, n);
 }
 
+// The tiled scatter for coloured tiles: the tile's points are taken colour
+// class by colour class (host-sorted), one point per lane; the points of one
+// class have disjoint tap footprints, so each box word takes at most one
+// plain read-modify-write per class -- no atomics, and the order of the
+// additions is fixed.  Same block order, boxes and flush as k_scatter.
+template <int KIND>
+__global__ void __launch_bounds__(kGatherThreads) k_scatter_col(Scatter g, const GTile* __restrict__ tiles) {
+    extern __shared__ float box[];
+    const int nt = blockDim.x;
+    const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
+    const int lbk = xcd_block(blockIdx.x, gridDim.x);
+    const int ng = gridDim.x / nzg;
+    const int zg = lbk % nzg, tile = (int)(((long long)(lbk / nzg) * g.tstride) % ng);
+    const GTile* tb = tiles + (size_t)tile * g.group;
+    const GTile t0 = tb[0];
+    const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
+    for (int z = z0; z < z1; ++z) {
+        const float* V = g.val + (size_t)z * g.v_z + (size_t)t0.span * g.v_span;
+        for (int gi = 0; gi < g.group; ++gi) {
+            const GTile t = tb[gi];
+            if (t.bw <= 0 || t.cr == 0) continue;   // uniform over the block
+            const int bs = t.bw | 1;
+            const int nbox = bs * t.bh;
+            const float2* cb = g.coord + (size_t)t.span * g.c_span;
+            for (int i = threadIdx.x; i < nbox; i += nt) box[i] = 0.f;
+            __syncthreads();
+            const int ncol = t.cr * t.cc;
+            for (int col = 0; col < ncol; ++col) {
+                const int pb = g.cls[t.cbase + col], pe = g.cls[t.cbase + col + 1];
+#pragma unroll 1
+                for (int i = pb + (int)threadIdx.x; i < pe; i += nt) {
+                    const int p = g.perm[t.pbase + i];
+                    const int r = p >> t.lnc, cidx = p & (t.nc - 1);
+                    const float2 c = cb[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
+                    float v = V[(size_t)(t0.r0 + r) * g.v_row + t0.c0 + cidx];
+                    if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
+                    const float fx = floorf(c.x), fy = floorf(c.y);
+                    const float ax = c.x - fx, ay = c.y - fy;
+                    const int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
+                    const int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
+                    float* q = box + ly * bs + lx;
+                    if (g.cubic) {
+                        float wx[4], wy[4];
+                        bspline_w(ax, wx);
+                        bspline_w(ay, wy);
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const float vb = wy[b] * v;
+                            float* qr = q + b * bs;
+                            const float s0 = qr[0], s1 = qr[1], s2 = qr[2], s3 = qr[3];
+                            qr[0] = s0 + wx[0] * vb;
+                            qr[1] = s1 + wx[1] * vb;
+                            qr[2] = s2 + wx[2] * vb;
+                            qr[3] = s3 + wx[3] * vb;
+                        }
+                    } else {
+                        float* q1 = q + bs + 1;
+                        float* q2 = q1 + bs;
+                        q1[0] += (1.f - ay) * (1.f - ax) * v;
+                        q1[1] += (1.f - ay) * ax * v;
+                        q2[0] += ay * (1.f - ax) * v;
+                        q2[1] += ay * ax * v;
+                    }
+                }
+                __syncthreads();
+            }
+            const float inv = 1.0f / (float)t.bw;
+            const int nb = t.bw * t.bh;
+            if (g.stage) {
+                float* B = g.stage + (size_t)z * g.stage_z + g.box_off[(size_t)tile * g.group + gi];
+                for (int i = threadIdx.x; i < nb; i += nt) {
+                    const int yy = (int)(((float)i + 0.5f) * inv);
+                    B[i] = box[yy * bs + (i - yy * t.bw)];
+                }
+            } else {
+                float* D = g.dst + (size_t)z * g.dst_z + (size_t)t.span * g.dst_span;
+                for (int i = threadIdx.x; i < nb; i += nt) {
+                    const int yy = (int)(((float)i + 0.5f) * inv);
+                    const int xx = i - yy * t.bw;
+                    const float v = box[yy * bs + xx];
+                    if (v != 0.f) unsafeAtomicAdd(D + (size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W), v);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 void launch_scatter(const Scatter& g0, const GTile* tiles, int ntiles, int kind, hipStream_t s) {
     if (ntiles <= 0) return;
     Scatter g = g0;
@@ -2116,12 +2224,190 @@ void launch_scatter(const Scatter& g0, const GTile* tiles, int ntiles, int kind,
     g.tstride = st;
     dim3 grid(ng * ((g.nz + g.zgroup - 1) / g.zgroup));
     size_t lds = (size_t)g.box_floats * sizeof(float);
+    if (g.perm) {
+        switch (kind) {
+            case 0: hipLaunchKernelGGL(k_scatter_col<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+            case 1: hipLaunchKernelGGL(k_scatter_col<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+            case 2: hipLaunchKernelGGL(k_scatter_col<2>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+            default: hipLaunchKernelGGL(k_scatter_col<3>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+        }
+    }
     switch (kind) {
         case 0: hipLaunchKernelGGL(k_scatter<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
         case 1: hipLaunchKernelGGL(k_scatter<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
         case 2: hipLaunchKernelGGL(k_scatter<2>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
         default: hipLaunchKernelGGL(k_scatter<3>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
     }
+}
+
+// Step A of the polar transpose: a block of 256 rows of one (span, angle);
+// the rows' detector runs are consecutive, so the coordinate and value loads
+// of neighbouring lanes are too.  u [nz][3][nA][Nrho].
+__global__ void __launch_bounds__(256) k_polarT_rows(PolarT p) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int al = blockIdx.y, k = blockIdx.z;
+    if (r >= p.Nr) return;
+    const int a0 = p.abeg[k], na = p.abeg[3 + k] - a0;
+    float* U = p.u + ((size_t)k * p.nA + al) * p.Nr + r;
+    const size_t u_z = (size_t)3 * p.nA * p.Nr;
+    if (al >= na) {
+        for (int z = 0; z < p.nz; ++z) U[z * u_z] = 0.f;
+        return;
+    }
+    const int a = a0 + al;
+    const int* xa = p.xs + (size_t)a * (p.Q + 1);
+    const float2* ca = p.coord + (size_t)a * p.c_row + p.c_co;
+    // unwrapped rows q = r (mod Nrho) inside the table's range; an angle's
+    // taps wrap around rho at most once, so at most two q have cells
+    int x0[2], x1[2], qq[2], nq = 0;
+    for (int q = p.qlo + (((r - p.qlo) % p.Nr) + p.Nr) % p.Nr; q <= p.qlo + p.Q && nq < 2; q += p.Nr) {
+        const int i = q - p.qlo;
+        const int lo = p.cubic ? i - 2 : i - 1, hi = p.cubic ? i + 2 : i + 1;
+        const int b0 = xa[max(lo, 0)], b1 = xa[min(hi, p.Q)];
+        if (b0 < b1) { x0[nq] = b0; x1[nq] = b1; qq[nq] = q; ++nq; }
+    }
+    for (int z = 0; z < p.nz; ++z) {
+        const float* V = p.val + (size_t)z * p.v_z + (size_t)a * p.N0;
+        float acc = 0.f;
+        for (int t = 0; t < nq; ++t)
+            for (int x = x0[t]; x < x1[t]; ++x) {
+                const float y = ca[x].y;
+                const float fy = floorf(y);
+                const int b = qq[t] - (int)fy + (p.cubic ? 1 : 0);
+                const float ay = y - fy;
+                float w;
+                if (p.cubic) {
+                    float wy[4];
+                    bspline_w(ay, wy);
+                    w = b == 0 ? wy[0] : b == 1 ? wy[1] : b == 2 ? wy[2] : wy[3];
+                } else {
+                    w = b == 0 ? 1.f - ay : ay;
+                }
+                acc += w * V[x];
+            }
+        U[z * u_z] = acc;
+    }
+}
+
+// Step B: a block owns 64 columns x 64 rows of one span; the u rows of the
+// angles whose taps reach those columns are staged through LDS 128 angles at
+// a time, and each lane sums its column for 16 rows.
+__global__ void __launch_bounds__(256) k_polarT_cols(PolarT p) {
+    __shared__ float us[128][65];
+    const int k = blockIdx.z, r0 = blockIdx.y * 64, jj0 = blockIdx.x * 64;
+    const int lc = threadIdx.x & 63, lr = threadIdx.x >> 6;
+    const int jj = jj0 + lc, j = p.jlo + jj;
+    const int* A = p.aj + (size_t)k * (p.jn + 5);
+    const int jl = min(jj, p.jn - 1);
+    const int lo = jj < p.jn ? (p.cubic ? A[jl] : A[jl + 1]) : 0;
+    const int hi = jj < p.jn ? (p.cubic ? A[jl + 4] : A[jl + 3]) : 0;
+    const int jlast = min(jj0 + 63, p.jn - 1);
+    const int blo = p.cubic ? A[jj0] : A[jj0 + 1], bhi = p.cubic ? A[jlast + 4] : A[jlast + 3];
+    const int a0 = p.abeg[k];
+    const size_t u_z = (size_t)3 * p.nA * p.Nr;
+    const size_t g_z = (size_t)p.Nr * p.Nt;
+    for (int z = 0; z < p.nz; ++z) {
+        float acc[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) acc[m] = 0.f;
+        const float* U = p.u + z * u_z + ((size_t)k * p.nA) * p.Nr;
+        for (int c0 = blo; c0 < bhi; c0 += 128) {
+            const int cn = min(128, bhi - c0);
+            __syncthreads();
+            for (int i = threadIdx.x; i < cn * 64; i += blockDim.x) {
+                const int ai = i >> 6, ri = i & 63;
+                const int rr = r0 + ri;
+                us[ai][ri] = rr < p.Nr ? U[(size_t)(c0 + ai) * p.Nr + rr] : 0.f;
+            }
+            __syncthreads();
+            for (int al = max(lo, c0); al < min(hi, c0 + cn); ++al) {
+                const float x = p.cxa[a0 + al];
+                const float fx = floorf(x);
+                const int c = j - (int)fx + (p.cubic ? 1 : 0);
+                const float ax = x - fx;
+                float w;
+                if (p.cubic) {
+                    float wx[4];
+                    bspline_w(ax, wx);
+                    w = c == 0 ? wx[0] : c == 1 ? wx[1] : c == 2 ? wx[2] : wx[3];
+                } else {
+                    w = c == 0 ? 1.f - ax : ax;
+                }
+#pragma unroll
+                for (int m = 0; m < 16; ++m) acc[m] += w * us[al - c0][lr + 4 * m];
+            }
+        }
+        if (jj < p.jn) {
+            float* Gk = p.G + ((size_t)k * p.nz + z) * g_z + j;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int rr = r0 + lr + 4 * m;
+                if (rr < p.Nr) Gk[(size_t)rr * p.Nt] = acc[m];
+            }
+        }
+    }
+}
+
+void launch_polar_transpose(const PolarT& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_polarT_rows, dim3((p.Nr + 255) / 256, p.nA, 3), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_polarT_cols, dim3((p.jn + 63) / 64, (p.Nr + 63) / 64, 3), dim3(256), 0, s, p);
+}
+
+// One block per destination tile of 32 x 32 points (4 per thread); the
+// tile's box list is walked once per 4 slices.
+__global__ void __launch_bounds__(256) k_box_sum(BoxSum b) {
+    constexpr int ZC = 4;
+    const int t = blockIdx.x;
+    const int plane = t / (b.tiles_x * b.tiles_y);
+    const int tt = t - plane * b.tiles_x * b.tiles_y;
+    const int ty = tt / b.tiles_x, tx = tt - ty * b.tiles_x;
+    const int X = tx * 32 + (threadIdx.x & 31);
+    const int Y0 = ty * 32 + (threadIdx.x >> 5);   // rows Y0 + 8 e, e < 4
+    const int beg = b.tile_beg[t], end = b.tile_beg[t + 1];
+    for (int z0 = 0; z0 < b.nz; z0 += ZC) {
+        const int zn = min(ZC, b.nz - z0);
+        float acc[ZC][4];
+#pragma unroll
+        for (int q = 0; q < ZC; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[q][e] = 0.f;
+        const float* St = b.stage + (size_t)z0 * b.stage_z;
+        for (int i = beg; i < end; ++i) {
+            const int id = b.tile_box[i];
+            const int4 r = b.box_rect[id];
+            const float* B = St + b.box_off[id];
+            // box origins lie within one period of the plane: one wrap at most
+            int dx = X - r.x;
+            dx += dx < 0 ? b.W : (dx >= b.W ? -b.W : 0);
+            if (dx >= r.z) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int dy = Y0 + 8 * e - r.y;
+                dy += dy < 0 ? b.H : (dy >= b.H ? -b.H : 0);
+                if (dy < r.w) {
+                    const float* Bq = B + dy * r.z + dx;
+#pragma unroll
+                    for (int q = 0; q < ZC; ++q)
+                        if (q < zn) acc[q][e] += Bq[(size_t)q * b.stage_z];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < ZC; ++q) {
+            if (q >= zn) break;
+            float* D = b.dst + (size_t)(z0 + q) * b.dst_z + (size_t)plane * b.dst_plane;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int Y = Y0 + 8 * e;
+                if (X < b.W && Y < b.H) D[(size_t)Y * b.W + X] = acc[q][e];
+            }
+        }
+    }
+}
+
+void launch_box_sum(const BoxSum& b, hipStream_t s) {
+    const int nt = b.nplanes * b.tiles_x * b.tiles_y;
+    if (nt > 0) hipLaunchKernelGGL(k_box_sum, dim3(nt), dim3(256), 0, s, b);
 }
 
 void launch_scatter_extra(const float* F, float* dst, int nz, int Ntheta, int Nrho, int W, int H, const int* xrow,

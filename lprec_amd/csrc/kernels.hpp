@@ -98,6 +98,13 @@ struct GTile {
     int r0, c0, nr, nc;
     int x0, y0, bw, bh;
     int span, lnc;
+    // colour classes of the transposed (scatter) use: point (r, c) has colour
+    // (r mod cr) * cc + (c mod cc), and points of one colour have disjoint
+    // 4 x 4 tap footprints; cr = 0: no such colouring (atomic accumulation).
+    // The tile's points with a sample, sorted by colour, are perm[pbase ..]
+    // (r * nc + c), colour k at positions cls[cbase + k] .. cls[cbase + k + 1].
+    int cr, cc;
+    int pbase, cbase;
 };
 // out[z*o_z + span*o_span + r*o_row + c] = (sum over the `group` tiles of a
 // block of) sample(src + z*src_z + span*src_span, coord[span*c_span +
@@ -143,8 +150,58 @@ struct Scatter {
     int v_row;
     int nz, zgroup, group, cubic, box_floats;
     int tstride;   // set by launch_scatter: block order over the tile groups
+    // staged flush (no atomics into dst): when stage is set, each block writes
+    // its accumulated source box, bw x bh row-major, to
+    // stage + z * stage_z + box_off[tile * group + gi]; launch_box_sum then
+    // forms dst from the staged boxes
+    float* stage;
+    long long stage_z;
+    const int* box_off;
+    const unsigned short* perm;   // colour-sorted points of the tiles (GTile.pbase)
+    const int* cls;               // colour class bounds (GTile.cbase)
 };
+// Destination of staged scatter boxes: dst tiles of 32 x 32 points, each the
+// sum, in increasing box order, of the staged boxes that cover it (periodic
+// placement); every dst point is written.
+struct BoxSum {
+    const float* stage;
+    long long stage_z;
+    const int* tile_beg;      // [ntiles + 1]: boxes of destination tile t in tile_box[tile_beg[t] ..)
+    const int* tile_box;
+    const int4* box_rect;     // per box: x0, y0 (unwrapped), bw, bh
+    const int* box_off;       // per box: offset in a slice's staging
+    float* dst;
+    long long dst_z, dst_plane;
+    int W, H, tiles_x, tiles_y, nplanes, nz;
+};
+void launch_box_sum(const BoxSum& b, hipStream_t s);
 void launch_scatter(const Scatter& g, const GTile* tiles, int ntiles, int kind, hipStream_t s);
+
+// Transpose of the log-polar -> polar gather as two gathers.  The polar map
+// is separable: an angle's samples share one theta coordinate, and their rho
+// coordinate grows with the detector cell.  Step A: u[z][k][a][r] = sum over
+// the detector cells x of angle a whose rho taps include row r of the rho
+// weight times R[z][a][x] (the cells form one run per (a, r)); step B:
+// G[k][z][r][j] = sum over the angles of span k whose theta taps include
+// column j of the theta weight times u[z][k][a][r].  Every output is summed
+// in a fixed order by one thread: no atomics.
+struct PolarT {
+    const float* val;         // R [nz][P0][N0]
+    long long v_z;
+    const float2* coord;      // coordinate of (a, x): coord[a * c_row + c_co + x]
+    int c_row, c_co;
+    const float* cxa;         // theta coordinate of each kept angle
+    float* u;                 // [nz][3][nA][Nr]
+    int nA;                   // angles per span (padded)
+    const int* abeg;          // [6] kept angles of span k: [abeg[k], abeg[3 + k])
+    const int* xs;            // [P0][Q + 1]: first x with floor(rho) >= qlo + i (N0 if none)
+    int qlo, Q;
+    float* G;                 // [3][nz][Nr][Nt]
+    const int* aj;            // [3][jn + 5]: first local angle of span k with floor(theta) >= jlo - 2 + i
+    int jlo, jn;              // step B writes columns [jlo, jlo + jn)
+    int Nr, Nt, N0, nz, cubic;
+};
+void launch_polar_transpose(const PolarT& p, hipStream_t s);
 // transpose of the extra samples of launch_rows_fwd: F [3][nz][Nrho][Ntheta]
 // values scattered into dst [nz][H][W] at the extras' coordinates
 void launch_scatter_extra(const float* F, float* dst, int nz, int Ntheta, int Nrho, int W, int H,
