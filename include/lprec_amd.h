@@ -122,8 +122,13 @@ int lp_info(const lp_handle* h, int64_t* info);
  * back 0 where it does not) or as the tiled scatter (0), key 8 = the tiled
  * scatters of lp_fwd_T / lp_adj_T flush their source boxes through a staging
  * area summed per destination tile (1, the default; no atomics into the
- * destination) or with global atomics (0).  Returns LP_ERR_ARG for bad
- * values. */
+ * destination) or with global atomics (0), key 9 = LDS row stride of the
+ * gather boxes, a 4-bit mask over the gathers image -> log-polar (bit 0),
+ * log-polar -> polar (1), sinogram -> log-polar (2), log-polar -> Cartesian
+ * (3): a set bit picks per box the least stride >= its width whose residue
+ * keeps same-bank box slots at least ~5 slots apart, a clear bit uses
+ * width | 1 (default 4; only before lp_init_fwd / lp_init_adj).  Returns
+ * LP_ERR_ARG for bad values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 

@@ -474,14 +474,15 @@ __device__ __forceinline__ void tile_rc(int p, int nc, float inv_nc, int& r, int
 }
 
 // Stage the box [x0, x0+bw) x [y0, y0+bh) of the periodic grid S [H][W] into
-// LDS rows of stride bw|1, UL loads in flight per lane.  A box inside the grid
-// (nearly all of them) needs no wrapping: element i = yy bw + xx sits at
-// source offset i + yy (W - bw) and LDS offset i + yy ((bw|1) - bw).
+// LDS rows of stride bs (>= bw, chosen on the host), UL loads in flight per
+// lane.  A box inside the grid (nearly all of them) needs no wrapping: element
+// i = yy bw + xx sits at source offset i + yy (W - bw) and LDS offset
+// i + yy (bs - bw).
 template <int UL>
 __device__ __forceinline__ void load_box(float* box, const float* __restrict__ S, const GTile& t, int W, int H) {
     constexpr int nt = kGatherThreads;
     const int nbox = t.bw * t.bh;
-    const int bs = t.bw | 1;
+    const int bs = t.bs;
     // (i + 0.5) / bw in float is exact enough for i, bw < 2^20
     const float inv = 1.0f / (float)t.bw;
     if (t.x0 >= 0 && t.y0 >= 0 && t.x0 + t.bw <= W && t.y0 + t.bh <= H) {
@@ -541,14 +542,14 @@ __device__ __forceinline__ void bspline_w6(float f, float* w) {
     w[3] = sq * f;
 }
 
-// One sample of the staged box (row stride bw|1) at source coordinate c (valid).
+// One sample of the staged box (row stride bs) at source coordinate c (valid).
 // The LDS slot of tap (0, 0) is formed in float from box-relative integers
 // (exact below 2^24) and clamped into the box in one step; the tile boxes hold
 // every tap of their points, so the clamp only guards the LDS bounds.
 __device__ __forceinline__ float box_sample(const float* box, const GTile& t, float2 c, int cubic) {
     const float fx = floorf(c.x), fy = floorf(c.y);
     const float ax = c.x - fx, ay = c.y - fy;
-    const int sy = t.bw | 1;
+    const int sy = t.bs;
     if (cubic) {
         float slot = fmaf(fy - (float)(t.y0 + 1), (float)sy, fx - (float)(t.x0 + 1));
         slot = __builtin_amdgcn_fmed3f(slot, 0.f, (float)((t.bh - 4) * sy + t.bw - 4));
@@ -572,13 +573,13 @@ __device__ __forceinline__ float box_sample(const float* box, const GTile& t, fl
 }
 
 // Stage the box of tile t like load_box, but, when the padded box (rows of
-// stride bs = bw | 1 including the pad column) lies inside the grid, with
+// stride bs including the pad columns) lies inside the grid, with
 // loads that write LDS directly, 4 bytes per lane: 64 consecutive box floats
 // are 64 consecutive LDS floats, no registers or LDS stores in between.  The
 // last chunk's lanes past the box reload its origin into LDS floats below the
 // next multiple of 64 (the launch rounds the box allocation up to that).
 __device__ __forceinline__ void stage_box(float* box, const float* __restrict__ S, const GTile& t, int W, int H) {
-    const int bs = t.bw | 1, nb = bs * t.bh;
+    const int bs = t.bs, nb = bs * t.bh;
     if (t.x0 >= 0 && t.y0 >= 0 && t.x0 + bs <= W && t.y0 + t.bh <= H) {
         const float* Sb = S + (size_t)t.y0 * W + t.x0;
         const float inv_bs = 1.0f / (float)bs;
@@ -616,7 +617,7 @@ __global__ void __launch_bounds__(kGatherThreads, 5) k_gather1(Gather g, const G
     const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
     const float2* cb = g.coord + (size_t)t.span * g.c_span + (size_t)(t.r0 + g.c_ro) * g.c_row + t.c0 + g.c_co;
     const float* sb = g.scale ? g.scale + (size_t)t.r0 * g.s_row + t.c0 : nullptr;
-    const int sy = t.bw | 1;
+    const int sy = t.bs;
     const int cubic = g.cubic;
     // box-relative origin of a sample's first tap and the largest valid slot
     const float ox = (float)(t.x0 + (cubic ? 1 : 0)), oy = (float)(t.y0 + (cubic ? 1 : 0));
@@ -2009,7 +2010,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter(Scatter g, const GTi
         for (int gi = 0; gi < g.group; ++gi) {
             const GTile t = tb[gi];
             if (t.bw <= 0 || (g.perm && t.cr > 0)) continue;   // uniform over the block
-            const int bs = t.bw | 1;
+            const int bs = t.bs;
             const int nbox = bs * t.bh;
             const float2* cbase = g.coord + (size_t)t.span * g.c_span;
             float2 cc[PPT];
@@ -2145,7 +2146,7 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter_col(Scatter g, const
         for (int gi = 0; gi < g.group; ++gi) {
             const GTile t = tb[gi];
             if (t.bw <= 0 || t.cr == 0) continue;   // uniform over the block
-            const int bs = t.bw | 1;
+            const int bs = t.bs;
             const int nbox = bs * t.bh;
             const float2* cb = g.coord + (size_t)t.span * g.c_span;
             for (int i = threadIdx.x; i < nbox; i += nt) box[i] = 0.f;

@@ -98,6 +98,8 @@ struct GTile {
     int r0, c0, nr, nc;
     int x0, y0, bw, bh;
     int span, lnc;
+    // LDS row stride of the staged box (>= bw)
+    int bs;
     // colour classes of the transposed (scatter) use: point (r, c) has colour
     // (r mod cr) * cc + (c mod cc), and points of one colour have disjoint
     // 4 x 4 tap footprints; cr = 0: no such colouring (atomic accumulation).

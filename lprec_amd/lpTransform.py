@@ -115,7 +115,8 @@ class lpTransform:
         """Create the per-GPU handle and upload the tables (lpTransform.py:34-40).
 
         ``schedule``: optional {knob: value} applied before the tables are
-        uploaded (see lprec_amd.tune.KNOBS; "boxcap" only takes effect here)."""
+        uploaded (see lprec_amd.tune.KNOBS; "boxcap" and "boxstride" only take
+        effect here)."""
         if not 0 <= int(gpu) < len(self.clphandle):
             raise ValueError("gpu index must be in [0, 16)")
         if not hasattr(self, "glpars"):

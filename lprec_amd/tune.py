@@ -17,8 +17,8 @@ import torch
 
 from ._lib import LprecError
 
-KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6}
-# knobs that can change after the tables are built (boxcap is fixed at init)
+KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6, "boxstride": 9}
+# knobs that can change after the tables are built (boxcap and boxstride are fixed at init)
 RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16", "fuse")
 SPACE = {
     "rows": (4, 8, 16),

@@ -84,3 +84,43 @@ def test_bubbles_fixture_backprojection(torch_cuda):
     print("bubbles: rel to oracle %.3e, fitted error to the image %.4f" % (rel(x, xm), err), flush=True)
     assert rel(x, xm) <= TOL
     assert err < 0.25
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("interp", ["cubic", "linear"])
+def test_box_stride_rule_keeps_results(torch_cuda, interp):
+    """The LDS row stride of the gather boxes (tuning key 9, fixed before the
+    tables are built) changes where box values sit in LDS, not the arithmetic:
+    fwd, adj and both transposes agree bit for bit between the width|1 rule
+    (mask 0) and the bank-spread rule on every gather (mask 15) -- fwd and
+    adj bit for bit.  A wider stride can split a tile whose box no longer
+    fits, which regroups the transposes' scatter sums: those are held to the
+    oracle parity bound, 1e-5 relative (measured 1.1e-6 for adj_T, whose
+    outputs carry cancellation)."""
+    torch = torch_cuda
+    N0, P0, ns = 256, 300, 3
+    outs = []
+    f = torch.from_numpy(phantom_stack(ns, N0, 12, seed=5)).cuda()
+    R = torch.from_numpy(np.random.default_rng(8).standard_normal((ns, P0, N0)).astype(np.float32)).cuda()
+    for mask in (0, 15):
+        lp = lpTransform(N0, P0, ns, "parzen", N0 / 2 - 3, interp)
+        lp.precompute(1)
+        lp.initcmem(1, 0, schedule={"boxstride": mask})
+        assert lp.clphandle[0].get_tuning(9) == mask
+        Ro = torch.zeros((ns, P0, N0), device="cuda")
+        fo = torch.zeros((ns, N0, N0), device="cuda")
+        fT = torch.zeros((ns, N0, N0), device="cuda")
+        RT = torch.zeros((ns, P0, N0), device="cuda")
+        lp.fwdp(Ro, f, 0)
+        lp.adjp(fo, R, 0)
+        lp.fwdp_transpose(fT, R, 0)
+        lp.adjp_transpose(RT, f, 0)
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy() for t in (Ro, fo, fT, RT)])
+    for name, a, b in zip(("fwd", "adj", "fwd_T", "adj_T"), outs[0], outs[1]):
+        d = float(np.max(np.abs(a - b)))
+        print("%s: max |stride 0 - stride 15| = %.3e" % (name, d), flush=True)
+        if name in ("fwd_T", "adj_T"):
+            assert rel(a, b) <= TOL
+        else:
+            assert d == 0.0
