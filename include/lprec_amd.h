@@ -127,8 +127,17 @@ int lp_info(const lp_handle* h, int64_t* info);
  * log-polar -> polar (1), sinogram -> log-polar (2), log-polar -> Cartesian
  * (3): a set bit picks per box the least stride >= its width whose residue
  * keeps same-bank box slots at least ~5 slots apart, a clear bit uses
- * width | 1 (default 4; only before lp_init_fwd / lp_init_adj).  Returns
- * LP_ERR_ARG for bad values. */
+ * width | 1 (default 4; only before lp_init_fwd / lp_init_adj), key 10 =
+ * row and column passes compiled at run time (hipRTC, at lp_init_fwd /
+ * lp_init_adj) for transform lengths the radix-16 kernels do not cover (1,
+ * the default; 0 = the generic mixed-radix kernels; reads back 1 only while
+ * such passes are loaded), key 11 = threads per block of the compiled column
+ * pass (256, 512 or 1024; 0 = automatic, the default; recompiles; reads back
+ * the threads in use), key 12 = variant of the compiled column pass (bit 0:
+ * the next slice's column is loaded while the current one is transformed;
+ * bit 1: twiddles are read per slice instead of held in registers; default
+ * 0, both measured slower at N = 2064; recompiles).  Returns LP_ERR_ARG for
+ * bad values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 

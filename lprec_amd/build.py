@@ -46,7 +46,8 @@ def build(verbose=False):
           os.path.join(CSRC, "precompute.cpp"), "-o", obj], verbose)
     objs.append(obj)
     tmp = OUT + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH] + objs + ["-o", tmp, "-lpthread"], verbose)
+    _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH] + objs + ["-o", tmp, "-lpthread", "-lhiprtc", "-ldl"],
+         verbose)
     os.replace(tmp, OUT)
     return OUT
 

@@ -1,0 +1,227 @@
+// Workgroup FFTs with every size known at compile time, for kernels that the
+// library compiles at run time (hipRTC) for the transform lengths of a handle
+// that the radix-16 register kernels do not cover (lengths with odd factors,
+// e.g. 2064 = 2^4 3 43 for a centre offset at N0 = 2048).
+//
+// The algorithm is the self-sorting Stockham FFT of fft_lds.hpp (same
+// butterflies, same twiddle products, same odd-radix conjugate-pair sums);
+// what changes is that the length M, the lines per block NL, their LDS stride
+// LDM, the thread count NT and the radix sequence are template arguments, so
+// each stage's butterfly count per thread, line and butterfly indices,
+// strides and twiddle steps are constants: no run-time divisions, no
+// per-stage guards for full rounds, and odd-radix weights read from one LDS
+// table built once per block.
+#pragma once
+#include "fft_lds.hpp"
+
+namespace lpct {
+
+template <int... Rs>
+struct Radices {};
+
+// Conjugate-pair weights of an odd radix R, forward direction:
+// wq[(q - 1) GP + p - 1] = exp(-2 pi i (p q mod R) / R), p, q = 1..(R-1)/2,
+// GP = ceil(((R-1)/2) / PP) PP columns (zero past (R-1)/2).
+template <int R>
+struct OddTable {
+    static constexpr int hR = (R - 1) / 2;
+    // output pairs per work item: even (the weights are read in pairs), at
+    // least 2 and at most 8
+    static constexpr int PP = hR <= 2 ? 2 : (hR <= 8 ? ((hR + 1) & ~1) : 8);
+    static constexpr int G = (hR + PP - 1) / PP;
+    static constexpr int GP = G * PP;
+    static constexpr int size = hR * GP;
+};
+
+// Fill the weight table of odd radix R (run by the whole block once; twM is
+// the length-M forward twiddle table, so W_R^m = twM[m M / R]).
+template <int M, int R>
+__device__ __forceinline__ void odd_table_fill(float2* wq, const float2* __restrict__ twM, int NT) {
+    using T = OddTable<R>;
+    for (int e = threadIdx.x; e < T::size; e += NT) {
+        const int q = e / T::GP + 1, p = e - (q - 1) * T::GP + 1;
+        wq[e] = p <= T::hR ? twM[((p * q) % R) * (M / R)] : make_float2(0.f, 0.f);
+    }
+}
+
+// One closed-form radix-R stage (R in 2, 3, 4, 5, 8) over NL lines of length
+// M at LDS stride LDM, Ns = product of the radices before it.
+template <int M, int NL, int LDM, int NT, int Ns, int R>
+__device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw, int dir) {
+    constexpr int nbf = M / R, total = NL * nbf, NB = (total + NT - 1) / NT;
+    constexpr int tstep = M / (Ns * R);
+    float2 v[NB][R];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int t = threadIdx.x + c * NT;
+        if ((c + 1) * NT <= total || t < total) {
+            const int b = NL == 1 ? 0 : t / nbf, i = t - b * nbf;
+            const int base = b * LDM + i;
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[c][q] = buf[lidx(base + q * nbf)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const int t = threadIdx.x + c * NT;
+        if ((c + 1) * NT <= total || t < total) {
+            const int b = NL == 1 ? 0 : t / nbf, i = t - b * nbf;
+            const int k = Ns == 1 ? 0 : i % Ns;
+            if (Ns > 1) {
+                // w^q from at most three table reads (w, w^2, w^4) and at most
+                // two products (the scheme of fft_stage)
+                const int base = k * tstep;
+                float2 wq[R];
+                wq[1] = twiddle(tw, base, dir);
+                if (R > 2) wq[2] = twiddle(tw, 2 * base, dir);
+                if (R > 3) wq[3] = cmul(wq[1], wq[2]);
+                if (R > 4) wq[4] = (R >= 8) ? twiddle(tw, 4 * base, dir) : cmul(wq[2], wq[2]);
+                if (R > 5) wq[5] = cmul(wq[1], wq[4]);
+                if (R > 6) wq[6] = cmul(wq[2], wq[4]);
+                if (R > 7) wq[7] = cmul(wq[3], wq[4]);
+#pragma unroll
+                for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], wq[q]);
+            }
+            dft_small<R>(v[c], dir);
+            const int dst = b * LDM + (i - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[lidx(dst + q * Ns)] = v[c][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Odd radix R > 5 by conjugate pairs (fft_stage_oddsym_t): a work item is one
+// butterfly and a group of PP output pairs; weights from the table of R.
+template <int M, int NL, int LDM, int NT, int Ns, int R>
+__device__ __forceinline__ void stage_odd(float2* buf, const float2* __restrict__ tw, const float2* __restrict__ wq,
+                                          int dir) {
+    using T = OddTable<R>;
+    constexpr int PP = T::PP, hR = T::hR, G = T::G, GP = T::GP;
+    constexpr int nbf = M / R, items = NL * nbf * G, NI = (items + NT - 1) / NT;
+    constexpr int tstep = M / (Ns * R);
+    const float bsign = dir < 0 ? 1.f : -1.f;   // the table holds the forward weights
+    float2 y0[NI], ya[NI][PP], yb[NI][PP];
+    int dsts[NI], gs[NI];
+#pragma unroll
+    for (int c = 0; c < NI; ++c) {
+        const int t = threadIdx.x + c * NT;
+        dsts[c] = 0;
+        gs[c] = 0;
+        y0[c] = make_float2(0.f, 0.f);
+        if ((c + 1) * NT <= items || t < items) {
+            const int bi = t / G, g = t - bi * G;
+            const int b = NL == 1 ? 0 : bi / nbf, i = bi - b * nbf;
+            const int k = Ns == 1 ? 0 : i % Ns;
+            const int src = b * LDM + i;
+            dsts[c] = b * LDM + (i - k) * R + k;
+            gs[c] = g;
+            const float2 x0 = buf[lidx(src)];
+            float2 A[PP], B[PP];
+#pragma unroll
+            for (int j = 0; j < PP; ++j) { A[j] = x0; B[j] = make_float2(0.f, 0.f); }
+            float2 S0 = x0;
+            const int e1 = k * tstep;
+            float2 w1 = make_float2(1.f, 0.f), w1c = w1, wa = w1, wb = w1;
+            if (Ns > 1) {
+                w1 = twiddle(tw, e1, dir);
+                w1c = cconj(w1);
+            }
+            const float4* wrow = reinterpret_cast<const float4*>(wq + g * PP);
+#pragma unroll 3
+            for (int q = 1; q <= hR; ++q) {
+                float2 xa = buf[lidx(src + q * nbf)], xb = buf[lidx(src + (R - q) * nbf)];
+                if (Ns > 1) {
+                    if (((q - 1) & 7) == 0) {
+                        wa = twiddle(tw, (q * e1) % M, dir);
+                        wb = twiddle(tw, ((R - q) * e1) % M, dir);
+                    }
+                    xa = cmul(xa, wa);
+                    xb = cmul(xb, wb);
+                    wa = cmul(wa, w1);
+                    wb = cmul(wb, w1c);
+                }
+                const float2 sq = cadd(xa, xb), dq = csub(xa, xb);
+                S0 = cadd(S0, sq);
+                const float4* wr = wrow + (q - 1) * (GP / 2);
+#pragma unroll
+                for (int jj = 0; jj < PP / 2; ++jj) {
+                    const float4 w2 = wr[jj];
+                    A[2 * jj].x += sq.x * w2.x; A[2 * jj].y += sq.y * w2.x;
+                    B[2 * jj].x += dq.x * w2.y; B[2 * jj].y += dq.y * w2.y;
+                    A[2 * jj + 1].x += sq.x * w2.z; A[2 * jj + 1].y += sq.y * w2.z;
+                    B[2 * jj + 1].x += dq.x * w2.w; B[2 * jj + 1].y += dq.y * w2.w;
+                }
+            }
+            y0[c] = S0;
+#pragma unroll
+            for (int j = 0; j < PP; ++j) {
+                const float2 iB = make_float2(-bsign * B[j].y, bsign * B[j].x);
+                ya[c][j] = cadd(A[j], iB);
+                yb[c][j] = csub(A[j], iB);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NI; ++c) {
+        const int t = threadIdx.x + c * NT;
+        if ((c + 1) * NT <= items || t < items) {
+            const int p0 = 1 + gs[c] * PP, dst = dsts[c];
+            if (gs[c] == 0) buf[lidx(dst)] = y0[c];
+#pragma unroll
+            for (int j = 0; j < PP; ++j) {
+                const int p = p0 + j;
+                if (p <= hR) {
+                    buf[lidx(dst + p * Ns)] = ya[c][j];
+                    buf[lidx(dst + (R - p) * Ns)] = yb[c][j];
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The odd radices of a plan, each with its weight table at a fixed offset of
+// one LDS array (tables for radices > 5 only).
+template <int... Rs>
+struct OddTables;
+template <>
+struct OddTables<> {
+    static constexpr int size = 0;
+    template <int M>
+    __device__ static void fill(float2*, const float2* __restrict__, int) {}
+};
+template <int R, int... Rest>
+struct OddTables<R, Rest...> {
+    static constexpr bool odd = R > 5 && (R & 1);
+    static constexpr int own = odd ? OddTable<R>::size : 0;
+    static constexpr int size = own + OddTables<Rest...>::size;
+    template <int M>
+    __device__ static void fill(float2* w, const float2* __restrict__ tw, int NT) {
+        if constexpr (odd) odd_table_fill<M, R>(w, tw, NT);
+        OddTables<Rest...>::template fill<M>(w + own, tw, NT);
+    }
+};
+
+// Full in-place FFT of NL lines (dir = -1 forward, +1 inverse, unnormalised);
+// wq: the tables of OddTables<Rs...>.  Ends with a barrier.
+template <int M, int NL, int LDM, int NT, int Ns, int R, int... Rest>
+__device__ __forceinline__ void run(float2* buf, const float2* __restrict__ tw, const float2* __restrict__ wq, int dir) {
+    if constexpr (R > 5 && (R & 1)) {
+        stage_odd<M, NL, LDM, NT, Ns, R>(buf, tw, wq, dir);
+        wq += OddTable<R>::size;
+    } else {
+        stage<M, NL, LDM, NT, Ns, R>(buf, tw, dir);
+    }
+    if constexpr (sizeof...(Rest) > 0) run<M, NL, LDM, NT, Ns * R, Rest...>(buf, tw, wq, dir);
+}
+
+template <int M, int NL, int LDM, int NT, int... Rs>
+__device__ __forceinline__ void fft(float2* buf, const float2* __restrict__ tw, const float2* __restrict__ wq,
+                                    int dir, Radices<Rs...>) {
+    run<M, NL, LDM, NT, 1, Rs...>(buf, tw, wq, dir);
+}
+
+}  // namespace lpct

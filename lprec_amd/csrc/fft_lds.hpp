@@ -8,7 +8,9 @@
 // (7, 11, 43, ...) runs through the generic O(R) per-output stage.  Twiddles
 // come from a table tw[j] = exp(-2 pi i j / M) built on the host in FP64.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 #define LP_MAX_STAGES 16
 

@@ -16,6 +16,9 @@
 #include <thread>
 #include <vector>
 
+#include <dlfcn.h>
+#include <hip/hiprtc.h>
+
 #include "fft64.hpp"
 #include "kernels.hpp"
 #include "lp_internal.hpp"
@@ -69,6 +72,16 @@ struct lp_handle {
     // separate pair (1.91 vs 1.90 ms per 32 slices at N = 2048)
     int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1, fuse_polar = 0;
     int polar_t = 1;   // separable polar transpose where the geometry allows (else the tiled scatter)
+    // row / column passes compiled at run time for lengths the radix-16
+    // kernels do not cover (rtc_build): setting, module, functions (rows
+    // fwd, rows inv, cols) and their launch shapes
+    int rtc_mode = 1;
+    int rtc_cols_req = 0;   // threads of the compiled column pass (0: automatic)
+    int rtc_cols_var = 0;   // its variant (k_cols_ct VAR: bit 0 prefetch, bit 1 reload twiddles)
+    bool rtc_tried = false;
+    hipModule_t rtc_mod = nullptr;
+    void* rtc_fn[3] = {nullptr, nullptr, nullptr};
+    int rtc_R = 0, rtc_rows_nt = 0, rtc_cols_nt = 0;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -218,6 +231,7 @@ struct lp_handle {
     ~lp_handle() {
         DeviceGuard g(device);
         prof_clear();
+        if (rtc_mod) (void)hipModuleUnload(rtc_mod);
         for (void* p : allocs) (void)hipFree(p);
     }
 };
@@ -734,6 +748,161 @@ int polar_transpose_tables(lp_handle* h, const std::vector<float2>& p2lp, const 
     return LP_OK;
 }
 
+// ---- row / column passes compiled at run time ------------------------------
+// For lengths with odd factors (a centre offset makes N = 2064 = 2^4 3 43 at
+// N0 = 2048) the generic mixed-radix kernels spend most of their time on run-
+// time index arithmetic.  rtc_build instantiates rtc_kernels.hpp (the same
+// passes with every size a template argument) for this handle's lengths and
+// radix sequences with hipRTC, once per process and set of lengths, and loads
+// the code object on the handle's device.  Failure is not an error: the
+// generic kernels stay in use (LPREC_AMD_VERBOSE=1 prints why).
+
+// directory of the device sources next to the shared library
+std::string csrc_dir() {
+    Dl_info info{};
+    if (!dladdr((void*)&lp_create, &info) || !info.dli_fname) return "";
+    std::string p(info.dli_fname);
+    const size_t k = p.find_last_of('/');
+    return (k == std::string::npos ? std::string(".") : p.substr(0, k)) + "/csrc";
+}
+
+std::string radix_list(const FftPlan& p) {
+    std::string r;
+    for (int i = 0; i < p.nst; ++i) r += ", " + std::to_string(p.radix[i]);
+    return r;
+}
+
+struct RtcCode {
+    std::vector<char> code;
+    std::vector<std::string> lowered;
+};
+std::mutex g_rtc_mu;
+std::map<std::string, std::shared_ptr<RtcCode>> g_rtc_cache;
+
+std::shared_ptr<RtcCode> rtc_compile(const std::string& arch, const std::vector<std::string>& names, std::string& err) {
+    std::string key = arch;
+    for (const auto& n : names) key += ";" + n;
+    std::lock_guard<std::mutex> lock(g_rtc_mu);
+    auto it = g_rtc_cache.find(key);
+    if (it != g_rtc_cache.end()) return it->second;
+    const std::string dir = csrc_dir();
+    if (dir.empty()) { err = "cannot locate the device sources"; return nullptr; }
+    const std::string src = "#include \"rtc_kernels.hpp\"\n";
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "lprec_rtc.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        err = "hiprtcCreateProgram failed";
+        return nullptr;
+    }
+    for (const auto& n : names) hiprtcAddNameExpression(prog, n.c_str());
+    const std::string a = "--offload-arch=" + arch, inc = "-I" + dir;
+    // packed FP32 instructions off, as for the library's own kernels
+    const char* opts[] = {a.c_str(), "-O3", "-std=c++17", inc.c_str(),
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    if (hiprtcCompileProgram(prog, 8, opts) != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        err = "hipRTC compile failed: " + log.substr(0, 2000);
+        hiprtcDestroyProgram(&prog);
+        return nullptr;
+    }
+    auto out = std::make_shared<RtcCode>();
+    for (const auto& n : names) {
+        const char* low = nullptr;
+        if (hiprtcGetLoweredName(prog, n.c_str(), &low) != HIPRTC_SUCCESS || !low) {
+            err = "no lowered name for " + n;
+            hiprtcDestroyProgram(&prog);
+            return nullptr;
+        }
+        out->lowered.emplace_back(low);
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    out->code.resize(cs);
+    if (!cs || hiprtcGetCode(prog, out->code.data()) != HIPRTC_SUCCESS) {
+        err = "hiprtcGetCode failed";
+        hiprtcDestroyProgram(&prog);
+        return nullptr;
+    }
+    hiprtcDestroyProgram(&prog);
+    g_rtc_cache[key] = out;
+    return out;
+}
+
+bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+// threads of a mixed-radix pass over `points` values (as threads_for)
+int rtc_threads(long long points) {
+    int nt = 256;
+    while (points > 16LL * nt && nt < 1024) nt *= 2;
+    return nt;
+}
+
+void rtc_build(lp_handle* h) {
+    if (h->rtc_tried || !h->rtc_mode) return;
+    h->rtc_tried = true;
+    if (h->rtc_mod) {
+        (void)hipModuleUnload(h->rtc_mod);
+        h->rtc_mod = nullptr;
+        h->rtc_fn[0] = h->rtc_fn[1] = h->rtc_fn[2] = nullptr;
+    }
+    const bool rows = !pow2_in(h->h, 256, 4096), cols = !pow2_in(h->Nr, 256, 4096);
+    if (!rows && !cols) return;
+    const char* v = std::getenv("LPREC_AMD_VERBOSE");
+    const bool verbose = v && v[0] == '1';
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) return;
+    std::string arch(prop.gcnArchName);
+    arch = arch.substr(0, arch.find(':'));
+    const int R = std::min(h->R, 8), ldm = row_ldm(h, R);
+    const int rnt = rtc_threads((long long)R * h->h);
+    const int cnt = h->rtc_cols_req ? h->rtc_cols_req : rtc_threads(h->Nr);
+    const std::string hr = radix_list(h->plan_h), mr = radix_list(h->plan_M);
+    std::vector<std::string> names = {
+        "lpct::k_rows_fwd_ct<" + std::to_string(h->h) + ", " + std::to_string(R) + ", " + std::to_string(ldm) + ", " +
+            std::to_string(rnt) + hr + ">",
+        "lpct::k_rows_inv_ct<" + std::to_string(h->h) + ", " + std::to_string(R) + ", " + std::to_string(ldm) + ", " +
+            std::to_string(rnt) + hr + ">",
+        "lpct::k_cols_ct<" + std::to_string(h->Nr) + ", " + std::to_string(cnt) + ", 2, " +
+            std::to_string(h->rtc_cols_var) + mr + ">"};
+    std::string err;
+    auto code = rtc_compile(arch, names, err);
+    hipModule_t mod = nullptr;
+    if (code && hipModuleLoadData(&mod, code->code.data()) != hipSuccess) {
+        (void)hipGetLastError();
+        err = "hipModuleLoadData failed";
+        mod = nullptr;
+    }
+    void* fn[3] = {nullptr, nullptr, nullptr};
+    for (int i = 0; mod && i < 3; ++i) {
+        hipFunction_t f = nullptr;
+        if (hipModuleGetFunction(&f, mod, code->lowered[i].c_str()) != hipSuccess) {
+            (void)hipGetLastError();
+            err = "hipModuleGetFunction failed for " + names[i];
+            (void)hipModuleUnload(mod);
+            mod = nullptr;
+            break;
+        }
+        fn[i] = (void*)f;
+    }
+    if (!mod) {
+        if (verbose) std::fprintf(stderr, "lprec_amd: run-time compiled passes unavailable (%s)\n", err.c_str());
+        return;
+    }
+    h->rtc_mod = mod;
+    // a pass whose length the radix-16 kernels cover keeps them
+    h->rtc_fn[0] = rows ? fn[0] : nullptr;
+    h->rtc_fn[1] = rows ? fn[1] : nullptr;
+    h->rtc_fn[2] = cols ? fn[2] : nullptr;
+    h->rtc_R = R;
+    h->rtc_rows_nt = rnt;
+    h->rtc_cols_nt = cnt;
+    if (verbose)
+        std::fprintf(stderr, "lprec_amd: run-time compiled passes: %s | %s | %s\n", names[0].c_str(), names[1].c_str(),
+                     names[2].c_str());
+}
+
 }  // namespace
 
 extern "C" {
@@ -946,6 +1115,7 @@ int lp_init_fwd(lp_handle* h, const int32_t* pi, const float* pf) {
             return rc;
     }
     h->has_fwd = true;
+    rtc_build(h);
     return LP_OK;
 }
 
@@ -1104,6 +1274,7 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         h->has_filter = true;
     }
     h->has_adj = true;
+    rtc_build(h);
     return LP_OK;
 }
 
@@ -1157,6 +1328,13 @@ static lpk::RowPass row_pass(lp_handle* h, int nz) {
     rp.fft16 = h->fft16;
     rp.in_rows = nullptr;
     rp.out_rows = nullptr;
+    const bool rtc = h->rtc_mode && h->rtc_mod;
+    rp.rtc_rows_fwd = rtc ? h->rtc_fn[0] : nullptr;
+    rp.rtc_rows_inv = rtc ? h->rtc_fn[1] : nullptr;
+    rp.rtc_cols = rtc ? h->rtc_fn[2] : nullptr;
+    rp.rtc_R = h->rtc_R;
+    rp.rtc_rows_nt = h->rtc_rows_nt;
+    rp.rtc_cols_nt = h->rtc_cols_nt;
     return rp;
 }
 
@@ -1541,6 +1719,35 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             if (value < 1024 || value > 32768) return h->fail(LP_ERR_ARG, "gather box budget must be in [1024, 32768] floats");
             h->box_cap = value;
             return LP_OK;
+        case 11:
+            if (value != 0 && value != 256 && value != 512 && value != 1024)
+                return h->fail(LP_ERR_ARG, "threads of the compiled column pass must be 0, 256, 512 or 1024");
+            if ((long long)value * 32 < h->Nr && value)
+                return h->fail(LP_ERR_ARG, "too few threads for the column length");
+            h->rtc_cols_req = value;
+            h->rtc_tried = false;
+            if (h->rtc_mode && (h->has_fwd || h->has_adj)) {
+                DeviceGuard guard(h->device);
+                rtc_build(h);
+            }
+            return LP_OK;
+        case 12:
+            if (value < 0 || value > 3) return h->fail(LP_ERR_ARG, "compiled column pass variant must be in [0, 3]");
+            h->rtc_cols_var = value;
+            h->rtc_tried = false;
+            if (h->rtc_mode && (h->has_fwd || h->has_adj)) {
+                DeviceGuard guard(h->device);
+                rtc_build(h);
+            }
+            return LP_OK;
+        case 10:
+            if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "run-time compiled passes must be 0 or 1");
+            h->rtc_mode = value;
+            if (value && (h->has_fwd || h->has_adj)) {
+                DeviceGuard guard(h->device);
+                rtc_build(h);
+            }
+            return LP_OK;
         case 9:
             if (h->has_fwd || h->has_adj)
                 return h->fail(LP_ERR_ARG, "the gather box stride is fixed once lp_init_fwd / lp_init_adj ran");
@@ -1619,6 +1826,9 @@ int lp_get_tuning(const lp_handle* h, int key, int* value) {
         case 7: *value = h->pt_ok ? h->polar_t : 0; return LP_OK;
         case 8: *value = h->staged; return LP_OK;
         case 9: *value = h->box_rule; return LP_OK;
+        case 10: *value = h->rtc_mode && h->rtc_mod ? 1 : 0; return LP_OK;
+        case 11: *value = h->rtc_cols_nt; return LP_OK;
+        case 12: *value = h->rtc_cols_var; return LP_OK;
     }
     return LP_ERR_ARG;
 }

@@ -1695,10 +1695,16 @@ void launch_rows_fwd(const RowPass& rp0, const float* F, const int2* range, hipS
     RowPass rg = rp;
     rg.R = std::min(rg.R, 8);
     rg = fit_zgroup(rg, 3LL * (rg.Nrho / rg.R));
+    dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
+    if (rg.rtc_rows_fwd && rg.R == rg.rtc_R) {
+        void* args[] = {&rg, &F, &range};
+        (void)hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_fwd, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
+                                    args, nullptr);
+        return;
+    }
     int nt = threads_for((long long)rg.R * rg.h);
     int ept = ept_for((long long)rg.R * rg.h, nt);
     int ldm = rg.ldm;
-    dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
     LP_DISPATCH_EPT(ept, k_rows_fwd, grid, dim3(nt), rows_lds_bytes(ldm, rg.R), s, rg, F, range, ldm);
 }
 
@@ -1731,10 +1737,16 @@ void launch_rows_inv(const RowPass& rp0, float* G, const int2* wrange, hipStream
     RowPass rg = rp;
     rg.R = std::min(rg.R, 8);
     rg = fit_zgroup(rg, 3LL * (rg.Nrho / rg.R));
+    dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
+    if (rg.rtc_rows_inv && rg.R == rg.rtc_R) {
+        void* args[] = {&rg, &G, &wrange};
+        (void)hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_inv, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
+                                    args, nullptr);
+        return;
+    }
     int nt = threads_for((long long)rg.R * rg.h);
     int ept = ept_for((long long)rg.R * rg.h, nt);
     int ldm = rg.ldm;
-    dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
     LP_DISPATCH_EPT(ept, k_rows_inv, grid, dim3(nt), rows_lds_bytes(ldm, rg.R), s, rg, G, wrange, ldm);
 }
 
@@ -1783,9 +1795,16 @@ void launch_cols(const RowPass& rp0, const FftPlan& planM, const float2* twM, co
         }
         return;
     }
+    dim3 grid(rp.h, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
+    if (rp.rtc_cols) {
+        RowPass rc = rp;
+        void* args[] = {&rc, &twM, &fzT};
+        (void)hipModuleLaunchKernel((hipFunction_t)rc.rtc_cols, grid.x, grid.y, grid.z, rc.rtc_cols_nt, 1, 1, 0, s, args,
+                                    nullptr);
+        return;
+    }
     int nt = threads_for(rp.Nrho);
     int ept = ept_for(rp.Nrho, nt);
-    dim3 grid(rp.h, 3, (rp.nz + rp.zgroup - 1) / rp.zgroup);
     LP_DISPATCH_EPT(ept, k_cols, grid, dim3(nt), cols_lds_bytes(rp.Nrho), s, rp, planM, twM, fzT);
 }
 

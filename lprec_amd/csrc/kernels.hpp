@@ -1,6 +1,12 @@
 // Device kernels of the log-polar pipeline (see kernels.hip).
 #pragma once
+#ifdef __HIPCC_RTC__
+// compiled at run time (rtc_kernels.hpp): hipRTC supplies the device runtime;
+// the launch declarations below only need the stream type
+typedef struct ihipStream_t* hipStream_t;
+#else
 #include <hip/hip_runtime.h>
+#endif
 
 #include "fft_lds.hpp"
 
@@ -58,6 +64,15 @@ struct RowPass {
     // the inverse row pass skips blocks with none.
     const int2* in_rows;
     const unsigned char* out_rows;
+    // Row and column passes compiled at run time for this handle's lengths
+    // (hipFunction_t of rtc_kernels.hpp; null: the mixed-radix kernels run):
+    // rows_fwd / rows_inv with rtc_R rows and rtc_rows_nt threads per block,
+    // cols with rtc_cols_nt threads.  Used where the radix-16 kernels do not
+    // apply.
+    void* rtc_rows_fwd;
+    void* rtc_rows_inv;
+    void* rtc_cols;
+    int rtc_R, rtc_rows_nt, rtc_cols_nt;
 };
 // R2C along theta of the gathered log-polar grids F [3][nz][Nrho][Ntheta];
 // range [3][Nrho]: [lo, hi) theta bounds of the non-empty points of each row
