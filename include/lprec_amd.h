@@ -133,11 +133,13 @@ int lp_info(const lp_handle* h, int64_t* info);
  * the default; 0 = the generic mixed-radix kernels; reads back 1 only while
  * such passes are loaded), key 11 = threads per block of the compiled column
  * pass (256, 512 or 1024; 0 = automatic, the default; recompiles; reads back
- * the threads in use), key 12 = variant of the compiled column pass (bit 0:
- * the next slice's column is loaded while the current one is transformed;
- * bit 1: twiddles are read per slice instead of held in registers; default
- * 0, both measured slower at N = 2064; recompiles).  Returns LP_ERR_ARG for
- * bad values. */
+ * the threads in use), key 12 = variant of the compiled passes, a 4-bit
+ * mask (bit 0: the column pass loads the next slice's column while the
+ * current one is transformed; bit 1: its index arithmetic is redone per slice
+ * instead of held in registers; bit 2: at least 4 waves per SIMD; bit 3:
+ * odd-radix work items of 4 output pairs instead of 8; default 14, the
+ * fastest measured at N = 2064; recompiles).  Returns LP_ERR_ARG for bad
+ * values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 

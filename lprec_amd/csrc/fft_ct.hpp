@@ -8,9 +8,10 @@
 // what changes is that the length M, the lines per block NL, their LDS stride
 // LDM, the thread count NT and the radix sequence are template arguments, so
 // each stage's butterfly count per thread, line and butterfly indices,
-// strides and twiddle steps are constants: no run-time divisions, no
-// per-stage guards for full rounds, and odd-radix weights read from one LDS
-// table built once per block.
+// strides and twiddle steps are constants (no run-time divisions, no
+// per-stage guards for full rounds), odd-radix weights come from one LDS
+// table built once per block, and the stages run out of place between two
+// LDS buffers (one barrier per stage, no outputs held across a barrier).
 #pragma once
 #include "fft_lds.hpp"
 
@@ -22,12 +23,15 @@ struct Radices {};
 // Conjugate-pair weights of an odd radix R, forward direction:
 // wq[(q - 1) GP + p - 1] = exp(-2 pi i (p q mod R) / R), p, q = 1..(R-1)/2,
 // GP = ceil(((R-1)/2) / PP) PP columns (zero past (R-1)/2).
+#ifndef LPCT_ODD_PP
+#define LPCT_ODD_PP 8
+#endif
 template <int R>
 struct OddTable {
     static constexpr int hR = (R - 1) / 2;
     // output pairs per work item: even (the weights are read in pairs), at
-    // least 2 and at most 8
-    static constexpr int PP = hR <= 2 ? 2 : (hR <= 8 ? ((hR + 1) & ~1) : 8);
+    // least 2 and at most LPCT_ODD_PP
+    static constexpr int PP = hR <= 2 ? 2 : (hR <= LPCT_ODD_PP ? ((hR + 1) & ~1) : LPCT_ODD_PP);
     static constexpr int G = (hR + PP - 1) / PP;
     static constexpr int GP = G * PP;
     static constexpr int size = hR * GP;
@@ -45,79 +49,68 @@ __device__ __forceinline__ void odd_table_fill(float2* wq, const float2* __restr
 }
 
 // One closed-form radix-R stage (R in 2, 3, 4, 5, 8) over NL lines of length
-// M at LDS stride LDM, Ns = product of the radices before it.
+// M at LDS stride LDM, Ns = product of the radices before it, from `in` to
+// `out` (two LDS buffers: each butterfly is read, transformed and written in
+// one pass, one barrier per stage).
 template <int M, int NL, int LDM, int NT, int Ns, int R>
-__device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw, int dir) {
+__device__ __forceinline__ void stage(const float2* in, float2* out, const float2* __restrict__ tw, int dir, int tid) {
     constexpr int nbf = M / R, total = NL * nbf, NB = (total + NT - 1) / NT;
     constexpr int tstep = M / (Ns * R);
-    float2 v[NB][R];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-        const int t = threadIdx.x + c * NT;
+        const int t = tid + c * NT;
         if ((c + 1) * NT <= total || t < total) {
             const int b = NL == 1 ? 0 : t / nbf, i = t - b * nbf;
             const int base = b * LDM + i;
+            float2 v[R];
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[c][q] = buf[lidx(base + q * nbf)];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        const int t = threadIdx.x + c * NT;
-        if ((c + 1) * NT <= total || t < total) {
-            const int b = NL == 1 ? 0 : t / nbf, i = t - b * nbf;
+            for (int q = 0; q < R; ++q) v[q] = in[lidx(base + q * nbf)];
             const int k = Ns == 1 ? 0 : i % Ns;
             if (Ns > 1) {
                 // w^q from at most three table reads (w, w^2, w^4) and at most
                 // two products (the scheme of fft_stage)
-                const int base = k * tstep;
+                const int e = k * tstep;
                 float2 wq[R];
-                wq[1] = twiddle(tw, base, dir);
-                if (R > 2) wq[2] = twiddle(tw, 2 * base, dir);
+                wq[1] = twiddle(tw, e, dir);
+                if (R > 2) wq[2] = twiddle(tw, 2 * e, dir);
                 if (R > 3) wq[3] = cmul(wq[1], wq[2]);
-                if (R > 4) wq[4] = (R >= 8) ? twiddle(tw, 4 * base, dir) : cmul(wq[2], wq[2]);
+                if (R > 4) wq[4] = (R >= 8) ? twiddle(tw, 4 * e, dir) : cmul(wq[2], wq[2]);
                 if (R > 5) wq[5] = cmul(wq[1], wq[4]);
                 if (R > 6) wq[6] = cmul(wq[2], wq[4]);
                 if (R > 7) wq[7] = cmul(wq[3], wq[4]);
 #pragma unroll
-                for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], wq[q]);
+                for (int q = 1; q < R; ++q) v[q] = cmul(v[q], wq[q]);
             }
-            dft_small<R>(v[c], dir);
+            dft_small<R>(v, dir);
             const int dst = b * LDM + (i - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[lidx(dst + q * Ns)] = v[c][q];
+            for (int q = 0; q < R; ++q) out[lidx(dst + q * Ns)] = v[q];
         }
     }
     __syncthreads();
 }
 
 // Odd radix R > 5 by conjugate pairs (fft_stage_oddsym_t): a work item is one
-// butterfly and a group of PP output pairs; weights from the table of R.
+// butterfly and a group of PP output pairs, weights from the table of R;
+// from `in` to `out`, one barrier.
 template <int M, int NL, int LDM, int NT, int Ns, int R>
-__device__ __forceinline__ void stage_odd(float2* buf, const float2* __restrict__ tw, const float2* __restrict__ wq,
-                                          int dir) {
+__device__ __forceinline__ void stage_odd(const float2* in, float2* out, const float2* __restrict__ tw,
+                                          const float2* __restrict__ wq, int dir, int tid) {
     using T = OddTable<R>;
     constexpr int PP = T::PP, hR = T::hR, G = T::G, GP = T::GP;
     constexpr int nbf = M / R, items = NL * nbf * G, NI = (items + NT - 1) / NT;
     constexpr int tstep = M / (Ns * R);
     const float bsign = dir < 0 ? 1.f : -1.f;   // the table holds the forward weights
-    float2 y0[NI], ya[NI][PP], yb[NI][PP];
-    int dsts[NI], gs[NI];
 #pragma unroll
     for (int c = 0; c < NI; ++c) {
-        const int t = threadIdx.x + c * NT;
-        dsts[c] = 0;
-        gs[c] = 0;
-        y0[c] = make_float2(0.f, 0.f);
+        const int t = tid + c * NT;
         if ((c + 1) * NT <= items || t < items) {
             const int bi = t / G, g = t - bi * G;
             const int b = NL == 1 ? 0 : bi / nbf, i = bi - b * nbf;
             const int k = Ns == 1 ? 0 : i % Ns;
             const int src = b * LDM + i;
-            dsts[c] = b * LDM + (i - k) * R + k;
-            gs[c] = g;
-            const float2 x0 = buf[lidx(src)];
+            const int dst = b * LDM + (i - k) * R + k;
+            const float2 x0 = in[lidx(src)];
             float2 A[PP], B[PP];
 #pragma unroll
             for (int j = 0; j < PP; ++j) { A[j] = x0; B[j] = make_float2(0.f, 0.f); }
@@ -131,7 +124,7 @@ __device__ __forceinline__ void stage_odd(float2* buf, const float2* __restrict_
             const float4* wrow = reinterpret_cast<const float4*>(wq + g * PP);
 #pragma unroll 3
             for (int q = 1; q <= hR; ++q) {
-                float2 xa = buf[lidx(src + q * nbf)], xb = buf[lidx(src + (R - q) * nbf)];
+                float2 xa = in[lidx(src + q * nbf)], xb = in[lidx(src + (R - q) * nbf)];
                 if (Ns > 1) {
                     if (((q - 1) & 7) == 0) {
                         wa = twiddle(tw, (q * e1) % M, dir);
@@ -154,28 +147,15 @@ __device__ __forceinline__ void stage_odd(float2* buf, const float2* __restrict_
                     B[2 * jj + 1].x += dq.x * w2.w; B[2 * jj + 1].y += dq.y * w2.w;
                 }
             }
-            y0[c] = S0;
-#pragma unroll
-            for (int j = 0; j < PP; ++j) {
-                const float2 iB = make_float2(-bsign * B[j].y, bsign * B[j].x);
-                ya[c][j] = cadd(A[j], iB);
-                yb[c][j] = csub(A[j], iB);
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NI; ++c) {
-        const int t = threadIdx.x + c * NT;
-        if ((c + 1) * NT <= items || t < items) {
-            const int p0 = 1 + gs[c] * PP, dst = dsts[c];
-            if (gs[c] == 0) buf[lidx(dst)] = y0[c];
+            const int p0 = 1 + g * PP;
+            if (g == 0) out[lidx(dst)] = S0;
 #pragma unroll
             for (int j = 0; j < PP; ++j) {
                 const int p = p0 + j;
                 if (p <= hR) {
-                    buf[lidx(dst + p * Ns)] = ya[c][j];
-                    buf[lidx(dst + (R - p) * Ns)] = yb[c][j];
+                    const float2 iB = make_float2(-bsign * B[j].y, bsign * B[j].x);
+                    out[lidx(dst + p * Ns)] = cadd(A[j], iB);
+                    out[lidx(dst + (R - p) * Ns)] = csub(A[j], iB);
                 }
             }
         }
@@ -205,23 +185,30 @@ struct OddTables<R, Rest...> {
     }
 };
 
-// Full in-place FFT of NL lines (dir = -1 forward, +1 inverse, unnormalised);
-// wq: the tables of OddTables<Rs...>.  Ends with a barrier.
+// Full FFT of NL lines (dir = -1 forward, +1 inverse, unnormalised) through
+// the LDS buffers a (holding the input) and b, alternating per stage;
+// returns the buffer that holds the result.  wq: the tables of
+// OddTables<Rs...>.  Ends with a barrier.
 template <int M, int NL, int LDM, int NT, int Ns, int R, int... Rest>
-__device__ __forceinline__ void run(float2* buf, const float2* __restrict__ tw, const float2* __restrict__ wq, int dir) {
+__device__ __forceinline__ float2* run(float2* a, float2* b, const float2* __restrict__ tw,
+                                       const float2* __restrict__ wq, int dir, int tid) {
     if constexpr (R > 5 && (R & 1)) {
-        stage_odd<M, NL, LDM, NT, Ns, R>(buf, tw, wq, dir);
+        stage_odd<M, NL, LDM, NT, Ns, R>(a, b, tw, wq, dir, tid);
         wq += OddTable<R>::size;
     } else {
-        stage<M, NL, LDM, NT, Ns, R>(buf, tw, dir);
+        stage<M, NL, LDM, NT, Ns, R>(a, b, tw, dir, tid);
     }
-    if constexpr (sizeof...(Rest) > 0) run<M, NL, LDM, NT, Ns * R, Rest...>(buf, tw, wq, dir);
+    if constexpr (sizeof...(Rest) > 0) return run<M, NL, LDM, NT, Ns * R, Rest...>(b, a, tw, wq, dir, tid);
+    else return b;
 }
 
+// tid: threadIdx.x, or an equal value the compiler cannot see as loop-
+// invariant (then the per-stage index arithmetic is redone per call instead of
+// being hoisted into registers held across a caller's loop)
 template <int M, int NL, int LDM, int NT, int... Rs>
-__device__ __forceinline__ void fft(float2* buf, const float2* __restrict__ tw, const float2* __restrict__ wq,
-                                    int dir, Radices<Rs...>) {
-    run<M, NL, LDM, NT, 1, Rs...>(buf, tw, wq, dir);
+__device__ __forceinline__ float2* fft(float2* a, float2* b, const float2* __restrict__ tw,
+                                       const float2* __restrict__ wq, int dir, Radices<Rs...>, int tid) {
+    return run<M, NL, LDM, NT, 1, Rs...>(a, b, tw, wq, dir, tid);
 }
 
 }  // namespace lpct

@@ -77,7 +77,9 @@ struct lp_handle {
     // fwd, rows inv, cols) and their launch shapes
     int rtc_mode = 1;
     int rtc_cols_req = 0;   // threads of the compiled column pass (0: automatic)
-    int rtc_cols_var = 0;   // its variant (k_cols_ct VAR: bit 0 prefetch, bit 1 reload twiddles)
+    // variant of the compiled passes: k_cols_ct VAR (bits 0-1), at least 4
+    // waves per SIMD (bit 2), odd-radix output pairs per item 4 instead of 8 (bit 3)
+    int rtc_cols_var = 14;
     bool rtc_tried = false;
     hipModule_t rtc_mod = nullptr;
     void* rtc_fn[3] = {nullptr, nullptr, nullptr};
@@ -779,8 +781,9 @@ struct RtcCode {
 std::mutex g_rtc_mu;
 std::map<std::string, std::shared_ptr<RtcCode>> g_rtc_cache;
 
-std::shared_ptr<RtcCode> rtc_compile(const std::string& arch, const std::vector<std::string>& names, std::string& err) {
-    std::string key = arch;
+std::shared_ptr<RtcCode> rtc_compile(const std::string& arch, int pp, const std::vector<std::string>& names,
+                                     std::string& err) {
+    std::string key = arch + ";" + std::to_string(pp);
     for (const auto& n : names) key += ";" + n;
     std::lock_guard<std::mutex> lock(g_rtc_mu);
     auto it = g_rtc_cache.find(key);
@@ -794,11 +797,11 @@ std::shared_ptr<RtcCode> rtc_compile(const std::string& arch, const std::vector<
         return nullptr;
     }
     for (const auto& n : names) hiprtcAddNameExpression(prog, n.c_str());
-    const std::string a = "--offload-arch=" + arch, inc = "-I" + dir;
+    const std::string a = "--offload-arch=" + arch, inc = "-I" + dir, dpp = "-DLPCT_ODD_PP=" + std::to_string(pp);
     // packed FP32 instructions off, as for the library's own kernels
-    const char* opts[] = {a.c_str(), "-O3", "-std=c++17", inc.c_str(),
+    const char* opts[] = {a.c_str(), "-O3", "-std=c++17", inc.c_str(), dpp.c_str(),
                           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    if (hiprtcCompileProgram(prog, 8, opts) != HIPRTC_SUCCESS) {
+    if (hiprtcCompileProgram(prog, 9, opts) != HIPRTC_SUCCESS) {
         size_t n = 0;
         hiprtcGetProgramLogSize(prog, &n);
         std::string log(n, '\0');
@@ -859,15 +862,17 @@ void rtc_build(lp_handle* h) {
     const int rnt = rtc_threads((long long)R * h->h);
     const int cnt = h->rtc_cols_req ? h->rtc_cols_req : rtc_threads(h->Nr);
     const std::string hr = radix_list(h->plan_h), mr = radix_list(h->plan_M);
+    const std::string wpe = (h->rtc_cols_var & 4) ? "4" : "2";
     std::vector<std::string> names = {
         "lpct::k_rows_fwd_ct<" + std::to_string(h->h) + ", " + std::to_string(R) + ", " + std::to_string(ldm) + ", " +
-            std::to_string(rnt) + hr + ">",
+            std::to_string(rnt) + ", " + wpe + ", 1" + hr + ">",
         "lpct::k_rows_inv_ct<" + std::to_string(h->h) + ", " + std::to_string(R) + ", " + std::to_string(ldm) + ", " +
-            std::to_string(rnt) + hr + ">",
-        "lpct::k_cols_ct<" + std::to_string(h->Nr) + ", " + std::to_string(cnt) + ", 2, " +
-            std::to_string(h->rtc_cols_var) + mr + ">"};
+            std::to_string(rnt) + ", " + wpe + ", 1" + hr + ">",
+        "lpct::k_cols_ct<" + std::to_string(h->Nr) + ", " + std::to_string(cnt) + ", " + wpe + ", " +
+            std::to_string(h->rtc_cols_var & 3) + mr + ">"};
     std::string err;
-    auto code = rtc_compile(arch, names, err);
+    const int pp = (h->rtc_cols_var & 8) ? 4 : 8;
+    auto code = rtc_compile(arch, pp, names, err);
     hipModule_t mod = nullptr;
     if (code && hipModuleLoadData(&mod, code->code.data()) != hipSuccess) {
         (void)hipGetLastError();
@@ -1732,7 +1737,7 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             }
             return LP_OK;
         case 12:
-            if (value < 0 || value > 3) return h->fail(LP_ERR_ARG, "compiled column pass variant must be in [0, 3]");
+            if (value < 0 || value > 15) return h->fail(LP_ERR_ARG, "compiled pass variant must be in [0, 15]");
             h->rtc_cols_var = value;
             h->rtc_tried = false;
             if (h->rtc_mode && (h->has_fwd || h->has_adj)) {

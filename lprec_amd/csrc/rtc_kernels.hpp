@@ -56,14 +56,15 @@ __device__ __forceinline__ int opaque_zero() {
 // kernel-spectrum product (column 0 packs the theta-DC and Nyquist columns),
 // inverse FFT.  Grid (h, 3, slice groups), NT threads, at least WPE waves per
 // SIMD.  VAR bit 0: the next slice's column is loaded into registers while
-// the current one is transformed; bit 1: twiddle tables are read per slice
-// (not kept in registers across slices).
+// the current one is transformed; bit 1: the FFT's index arithmetic and
+// twiddle reads are redone per slice (not hoisted into registers held across
+// the slices of a block).
 template <int M, int NT, int WPE, int VAR, int... Rs>
 __global__ void __launch_bounds__(NT, WPE) k_cols_ct(lpk::RowPass rp, const float2* __restrict__ twM,
                                                      const float2* __restrict__ fzT) {
-    constexpr bool PF = VAR & 1, RELOAD = (VAR & 2) != 0;
+    constexpr bool PF = VAR & 1, OPAQUE = (VAR & 2) != 0;
     constexpr int PL = (M + NT - 1) / NT;   // column values per thread
-    __shared__ float2 smc[lds_slots(M)];
+    __shared__ float2 smc[lds_slots(M)], smd[lds_slots(M)];
     __shared__ __align__(16) float2 wq[odd_slots<Rs...>()];
     OddTables<Rs...>::template fill<M>(wq, twM, NT);
     const int h = rp.h;
@@ -91,7 +92,7 @@ __global__ void __launch_bounds__(NT, WPE) k_cols_ct(lpk::RowPass rp, const floa
     __syncthreads();
     for (int z = z0; z < z1; ++z) {
         float2* col = column(z);
-        const float2* tw = RELOAD ? twM + opaque_zero() : twM;
+        const int tid = OPAQUE ? (int)threadIdx.x + opaque_zero() : (int)threadIdx.x;
         if (PF) {
 #pragma unroll
             for (int u = 0; u < PL; ++u) {
@@ -106,27 +107,27 @@ __global__ void __launch_bounds__(NT, WPE) k_cols_ct(lpk::RowPass rp, const floa
                 [&](int i, float2 v) { smc[lidx(i)] = v; });
             __syncthreads();
         }
-        fft<M, 1, M, NT>(smc, tw, wq, -1, Radices<Rs...>{});
+        float2* sp = fft<M, 1, M, NT>(smc, smd, twM, wq, -1, Radices<Rs...>{}, tid);
         if (f == 0) {
             for (int k = threadIdx.x; k <= M / 2; k += NT) {
                 const int km = k == 0 ? 0 : M - k;
-                const float2 A = smc[lidx(k)], Bc = cconj(smc[lidx(km)]);
+                const float2 A = sp[lidx(k)], Bc = cconj(sp[lidx(km)]);
                 const float2 D = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
                 const float2 E = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
                 const float2 Q = make_float2(E.y, -E.x);  // E / i
                 const float2 yk = cadd(cmul(D, fzval(fz, k, rp.conj)), rot90(cmul(Q, fzval(fzh, k, rp.conj)), 1));
                 const float2 ykm =
                     cadd(cmul(cconj(D), fzval(fz, km, rp.conj)), rot90(cmul(cconj(Q), fzval(fzh, km, rp.conj)), 1));
-                smc[lidx(k)] = yk;
-                if (km != k) smc[lidx(km)] = ykm;
+                sp[lidx(k)] = yk;
+                if (km != k) sp[lidx(km)] = ykm;
             }
         } else {
-            for (int i = threadIdx.x; i < M; i += NT) smc[lidx(i)] = cmul(smc[lidx(i)], fzval(fz, i, rp.conj));
+            for (int i = threadIdx.x; i < M; i += NT) sp[lidx(i)] = cmul(sp[lidx(i)], fzval(fz, i, rp.conj));
         }
         __syncthreads();
-        fft<M, 1, M, NT>(smc, tw, wq, +1, Radices<Rs...>{});
+        const float2* xp = fft<M, 1, M, NT>(sp, sp == smc ? smd : smc, twM, wq, +1, Radices<Rs...>{}, tid);
         for (int i = threadIdx.x; i < M; i += NT)
-            if (!om || om[i]) col[i] = smc[lidx(i)];
+            if (!om || om[i]) col[i] = xp[lidx(i)];
         __syncthreads();
     }
 }
@@ -134,11 +135,11 @@ __global__ void __launch_bounds__(NT, WPE) k_cols_ct(lpk::RowPass rp, const floa
 // Row pass, forward (k_rows_fwd): R log-polar rows of one span, real FFT of
 // length 2H along theta (complex FFT of length H + post-processing), half
 // spectra stored transposed into S.  Grid (Nrho / R, 3, slice groups).
-template <int H, int R, int LDM, int NT, int... Rs>
-__global__ void __launch_bounds__(NT) k_rows_fwd_ct(lpk::RowPass rp, const float* __restrict__ F,
+template <int H, int R, int LDM, int NT, int WPE, int OPQ, int... Rs>
+__global__ void __launch_bounds__(NT, WPE) k_rows_fwd_ct(lpk::RowPass rp, const float* __restrict__ F,
                                                     const int2* __restrict__ range) {
     const float2* __restrict__ twh = rp.twh;
-    __shared__ float2 smc[lds_slots(R * LDM)];
+    __shared__ float2 smc[lds_slots(R * LDM)], smd[lds_slots(R * LDM)];
     __shared__ __align__(16) float2 wq[odd_slots<Rs...>()];
     __shared__ int2 rng_s[R];
     OddTables<Rs...>::template fill<H>(wq, twh, NT);
@@ -169,12 +170,13 @@ __global__ void __launch_bounds__(NT) k_rows_fwd_ct(lpk::RowPass rp, const float
                 smc[lidx(r * LDM + e)] = v;
             });
         __syncthreads();
-        fft<H, R, LDM, NT>(smc, twh, wq, -1, Radices<Rs...>{});
-        r2c_post(smc, LDM, R, H, rp.twN);
+        float2* sp = fft<H, R, LDM, NT>(smc, smd, twh, wq, -1, Radices<Rs...>{},
+                                        (int)threadIdx.x + (OPQ ? opaque_zero() : 0));
+        r2c_post(sp, LDM, R, H, rp.twN);
         float2* out = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
         for (int q = threadIdx.x; q < R * H; q += NT) {
             const int fr = q / R, r = q - fr * R;
-            out[(size_t)fr * Nr + r] = smc[lidx(r * LDM + fr)];
+            out[(size_t)fr * Nr + r] = sp[lidx(r * LDM + fr)];
         }
         __syncthreads();
     }
@@ -182,11 +184,11 @@ __global__ void __launch_bounds__(NT) k_rows_fwd_ct(lpk::RowPass rp, const float
 
 // Row pass, inverse (k_rows_inv): C2R along theta for R rows, writing the
 // theta columns the output gather reads.
-template <int H, int R, int LDM, int NT, int... Rs>
-__global__ void __launch_bounds__(NT) k_rows_inv_ct(lpk::RowPass rp, float* __restrict__ G,
+template <int H, int R, int LDM, int NT, int WPE, int OPQ, int... Rs>
+__global__ void __launch_bounds__(NT, WPE) k_rows_inv_ct(lpk::RowPass rp, float* __restrict__ G,
                                                     const int2* __restrict__ wrange) {
     const float2* __restrict__ twh = rp.twh;
-    __shared__ float2 smc[lds_slots(R * LDM)];
+    __shared__ float2 smc[lds_slots(R * LDM)], smd[lds_slots(R * LDM)];
     __shared__ __align__(16) float2 wq[odd_slots<Rs...>()];
     OddTables<Rs...>::template fill<H>(wq, twh, NT);
     constexpr int Nt = 2 * H;
@@ -209,13 +211,14 @@ __global__ void __launch_bounds__(NT) k_rows_inv_ct(lpk::RowPass rp, float* __re
             });
         __syncthreads();
         c2r_pre(smc, LDM, R, H, rp.twN);
-        fft<H, R, LDM, NT>(smc, twh, wq, +1, Radices<Rs...>{});
+        const float2* xp = fft<H, R, LDM, NT>(smc, smd, twh, wq, +1, Radices<Rs...>{},
+                                              (int)threadIdx.x + (OPQ ? opaque_zero() : 0));
         float* out = G + (((size_t)span * rp.nz + z) * Nr + row0) * (size_t)Nt;
         for (int p = threadIdx.x; p < R * H; p += NT) {
             const int r = p / H, e = p - r * H;
             const int2 wr = wrange ? wrange[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
             if (2 * e + 1 >= wr.x && 2 * e < wr.y)
-                *reinterpret_cast<float2*>(out + (size_t)r * Nt + 2 * e) = smc[lidx(r * LDM + e)];
+                *reinterpret_cast<float2*>(out + (size_t)r * Nt + 2 * e) = xp[lidx(r * LDM + e)];
         }
         __syncthreads();
     }
