@@ -1698,9 +1698,10 @@ void launch_rows_fwd(const RowPass& rp0, const float* F, const int2* range, hipS
     dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
     if (rg.rtc_rows_fwd && rg.R == rg.rtc_R) {
         void* args[] = {&rg, &F, &range};
-        (void)hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_fwd, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
-                                    args, nullptr);
-        return;
+        if (hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_fwd, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
+                                  args, nullptr) == hipSuccess)
+            return;
+        (void)hipGetLastError();   // the generic kernel below does the pass
     }
     int nt = threads_for((long long)rg.R * rg.h);
     int ept = ept_for((long long)rg.R * rg.h, nt);
@@ -1740,9 +1741,10 @@ void launch_rows_inv(const RowPass& rp0, float* G, const int2* wrange, hipStream
     dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
     if (rg.rtc_rows_inv && rg.R == rg.rtc_R) {
         void* args[] = {&rg, &G, &wrange};
-        (void)hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_inv, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
-                                    args, nullptr);
-        return;
+        if (hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_inv, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
+                                  args, nullptr) == hipSuccess)
+            return;
+        (void)hipGetLastError();   // the generic kernel below does the pass
     }
     int nt = threads_for((long long)rg.R * rg.h);
     int ept = ept_for((long long)rg.R * rg.h, nt);
@@ -1799,9 +1801,10 @@ void launch_cols(const RowPass& rp0, const FftPlan& planM, const float2* twM, co
     if (rp.rtc_cols) {
         RowPass rc = rp;
         void* args[] = {&rc, &twM, &fzT};
-        (void)hipModuleLaunchKernel((hipFunction_t)rc.rtc_cols, grid.x, grid.y, grid.z, rc.rtc_cols_nt, 1, 1, 0, s, args,
-                                    nullptr);
-        return;
+        if (hipModuleLaunchKernel((hipFunction_t)rc.rtc_cols, grid.x, grid.y, grid.z, rc.rtc_cols_nt, 1, 1, 0, s, args,
+                                  nullptr) == hipSuccess)
+            return;
+        (void)hipGetLastError();   // the generic kernel below does the pass
     }
     int nt = threads_for(rp.Nrho);
     int ept = ept_for(rp.Nrho, nt);
