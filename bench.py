@@ -127,7 +127,8 @@ def pmc_summary(root, copy_mib):
             "stages": {k: [int(fetch[k] * ff), int(write.get(k, 0.0) * fw)] for k in fetch}}
 
 
-SOURCES = ("kernels.hip", "handle.hip", "fft_lds.hpp", "fft64.hpp", "kernels.hpp", "lp_internal.hpp")
+SOURCES = ("kernels.hip", "handle.hip", "fft_lds.hpp", "fft64.hpp", "kernels.hpp", "lp_internal.hpp", "fft_ct.hpp",
+           "rtc_kernels.hpp")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fwdadj_N2048_P1500_b32.json")
 
 
@@ -459,7 +460,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-kernel HBM ceiling")
-    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16, boxcap, fuse, boxstride, rtc, rtccols, rtcvar)")
+    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16, boxcap, fuse, boxstride, rtc, rtccols, rtcvar, pzgroup)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--pmc-summary", default="", metavar="DIR",
@@ -523,7 +524,7 @@ def main():
     N0, P0, ns = args.N, args.nproj, args.slices
     cor = N0 / 2
     knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6,
-             "boxstride": 9, "rtc": 10, "rtccols": 11, "rtcvar": 12}
+             "boxstride": 9, "rtc": 10, "rtccols": 11, "rtcvar": 12, "pzgroup": 13}
     sched = {}
     for kv in filter(None, args.tune.split(",")):
         k, v = kv.split("=")

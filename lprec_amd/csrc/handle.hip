@@ -72,6 +72,10 @@ struct lp_handle {
     // separate pair (1.91 vs 1.90 ms per 32 slices at N = 2048)
     int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1, fuse_polar = 0;
     int polar_t = 1;   // separable polar transpose where the geometry allows (else the tiled scatter)
+    // slices per block of the log-polar -> polar gather: its coordinate table
+    // ([P][N] float2) is read once per block, so more slices per block read
+    // it fewer times
+    int pzgroup = 8;
     // row / column passes compiled at run time for lengths the radix-16
     // kernels do not cover (rtc_build): setting, module, functions (rows
     // fwd, rows inv, cols) and their launch shapes
@@ -1412,7 +1416,7 @@ static int fwd_chain(lp_handle* h, float* R, const float* f, int ns0, hipStream_
         g.coord = h->d_p2lp; g.c_span = 0; g.c_row = h->os * N; g.c_ro = 0; g.c_co = N / 2 - h->cor;
         g.scale = nullptr; g.s_row = 0;
         g.out = R; g.o_z = (long long)h->P0 * h->N0; g.o_span = 0; g.o_row = h->N0;
-        g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxP;
+        g.nz = ns0; g.zgroup = h->pzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxP;
         lpk::launch_gather(g, h->d_tilesP, h->ntilesP, 2, s);
     }
     mark(7);
@@ -1736,6 +1740,10 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
                 rtc_build(h);
             }
             return LP_OK;
+        case 13:
+            if (value < 1) return h->fail(LP_ERR_ARG, "slices per polar gather block must be >= 1");
+            h->pzgroup = value;
+            return LP_OK;
         case 12:
             if (value < 0 || value > 15) return h->fail(LP_ERR_ARG, "compiled pass variant must be in [0, 15]");
             h->rtc_cols_var = value;
@@ -1834,6 +1842,7 @@ int lp_get_tuning(const lp_handle* h, int key, int* value) {
         case 10: *value = h->rtc_mode && h->rtc_mod ? 1 : 0; return LP_OK;
         case 11: *value = h->rtc_cols_nt; return LP_OK;
         case 12: *value = h->rtc_cols_var; return LP_OK;
+        case 13: *value = h->pzgroup; return LP_OK;
     }
     return LP_ERR_ARG;
 }

@@ -18,13 +18,14 @@ import torch
 from ._lib import LprecError
 
 KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6, "boxstride": 9,
-         "rtc": 10, "rtccols": 11, "rtcvar": 12}
+         "rtc": 10, "rtccols": 11, "rtcvar": 12, "pzgroup": 13}
 # knobs that can change after the tables are built (boxcap and boxstride are fixed at init)
-RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16", "fuse")
+RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16", "fuse", "pzgroup")
 SPACE = {
     "rows": (4, 8, 16),
     "zgroup": (1, 2, 4, 8),
     "gzgroup": (1, 2, 4, 8),
+    "pzgroup": (4, 8, 16),
     "colchunk": (32, 64, 128),
     "fft16": (1, 0),
     "fuse": (1, 0),
