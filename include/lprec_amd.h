@@ -139,7 +139,7 @@ int lp_info(const lp_handle* h, int64_t* info);
  * instead of held in registers; bit 2: at least 4 waves per SIMD; bit 3:
  * odd-radix work items of 4 output pairs instead of 8; default 14, the
  * fastest measured at N = 2064; recompiles), key 13 = slices per block of
- * lp_fwd's log-polar -> polar gather (>= 1, default 8; each block reads its
+ * lp_fwd's log-polar -> polar gather (>= 1, default 4; each block reads its
  * tile's coordinates once).  Returns LP_ERR_ARG for bad values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);

@@ -73,9 +73,11 @@ struct lp_handle {
     int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1, fuse_polar = 0;
     int polar_t = 1;   // separable polar transpose where the geometry allows (else the tiled scatter)
     // slices per block of the log-polar -> polar gather: its coordinate table
-    // ([P][N] float2) is read once per block, so more slices per block read
-    // it fewer times
-    int pzgroup = 8;
+    // ([P][N] float2) is read once per block.  8 measured 0.587 against 0.595
+    // ms per 32 slices at N = 2048 but read more of G from HBM (2119 against
+    // 2028 MB: the blocks of neighbouring tiles, which share G rows, run
+    // further apart), so the default stays at 4
+    int pzgroup = 4;
     // row / column passes compiled at run time for lengths the radix-16
     // kernels do not cover (rtc_build): setting, module, functions (rows
     // fwd, rows inv, cols) and their launch shapes

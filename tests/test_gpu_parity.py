@@ -124,3 +124,28 @@ def test_box_stride_rule_keeps_results(torch_cuda, interp):
             assert rel(a, b) <= TOL
         else:
             assert d == 0.0
+
+
+@pytest.mark.gpu
+def test_polar_gather_slices_per_block_keep_results(torch_cuda):
+    """Slices per block of the log-polar -> polar gather (tuning key 13) only
+    regroups work: every point is the same 16-tap sum, so fwd is bit-identical
+    for 1, 3 and 8 slices per block (a ragged last group included)."""
+    torch = torch_cuda
+    N0, P0, ns = 256, 300, 7
+    lp = lpTransform(N0, P0, ns, "none", N0 / 2, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    h = lp.clphandle[0]
+    f = torch.from_numpy(phantom_stack(ns, N0, 12, seed=4)).cuda()
+    outs = []
+    try:
+        for zg in (1, 3, 8):
+            h.set_tuning(13, zg)
+            R = torch.zeros((ns, P0, N0), device="cuda")
+            lp.fwdp(R, f, 0)
+            torch.cuda.synchronize()
+            outs.append(R.cpu().numpy())
+    finally:
+        h.set_tuning(13, 4)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])

@@ -85,3 +85,30 @@ def test_runtime_compiled_passes_match_oracle_at_2064(torch_cuda):
     ef, ea = rel(Rg, mirror.fwd(tab, f)), rel(fg, mirror.adj(tab, R))
     print("N 2064 compiled passes vs oracle: fwd %.2e adj %.2e" % (ef, ea), flush=True)
     assert ef <= 1e-5 and ea <= 1e-5
+
+
+def test_compiled_pass_variants_agree(torch_cuda):
+    """Every variant of the compiled passes (tuning key 12: prefetch, per-slice
+    index arithmetic, waves per SIMD, odd-radix items of 4 or 8 output pairs)
+    runs the same butterflies in the same order: fwd and adj agree with the
+    default variant to 1e-6 (N = 528)."""
+    torch = torch_cuda
+    N0, P0, ns = 512, 400, 3
+    f = phantom_stack(ns, N0, 12, seed=7)
+    R = np.random.default_rng(9).standard_normal((ns, P0, N0)).astype(np.float32)
+    lp = lpTransform(N0, P0, ns, "parzen", 253, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    h = lp.clphandle[0]
+    default = h.get_tuning(12)
+    ref = _run(lp, f, R, torch)
+    try:
+        for var in (0, 1, 3, 6, 15):
+            h.set_tuning(12, var)
+            assert h.get_tuning(10) == 1
+            a = _run(lp, f, R, torch)
+            ef, ea = rel(a[0], ref[0]), rel(a[1], ref[1])
+            print("variant %2d vs %d: fwd %.2e adj %.2e" % (var, default, ef, ea), flush=True)
+            assert ef <= 1e-6 and ea <= 1e-6
+    finally:
+        h.set_tuning(12, default)
