@@ -2155,6 +2155,10 @@ void launch_zero(float* p, size_t n, hipStream_t s) {
 template <int KIND>
 __global__ void __launch_bounds__(kGatherThreads) k_scatter_col(Scatter g, const GTile* __restrict__ tiles) {
     extern __shared__ float box[];
+    // KZ slices share each pass over the colour classes: a point's tap
+    // geometry is computed once and added into KZ boxes (one per slice, LDS
+    // KZ * box_floats), so the block has half the class barriers per slice
+    constexpr int KZ = kScatterSlices;
     const int nt = blockDim.x;
     const int nzg = (g.nz + g.zgroup - 1) / g.zgroup;
     const int lbk = xcd_block(blockIdx.x, gridDim.x);
@@ -2163,15 +2167,18 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter_col(Scatter g, const
     const GTile* tb = tiles + (size_t)tile * g.group;
     const GTile t0 = tb[0];
     const int z0 = zg * g.zgroup, z1 = min(z0 + g.zgroup, g.nz);
-    for (int z = z0; z < z1; ++z) {
-        const float* V = g.val + (size_t)z * g.v_z + (size_t)t0.span * g.v_span;
+    const int bstride = g.box_floats;
+    for (int zb = z0; zb < z1; zb += KZ) {
+        const int zn = min(KZ, z1 - zb);
+        const float* V = g.val + (size_t)zb * g.v_z + (size_t)t0.span * g.v_span;
         for (int gi = 0; gi < g.group; ++gi) {
             const GTile t = tb[gi];
             if (t.bw <= 0 || t.cr == 0) continue;   // uniform over the block
             const int bs = t.bs;
             const int nbox = bs * t.bh;
             const float2* cb = g.coord + (size_t)t.span * g.c_span;
-            for (int i = threadIdx.x; i < nbox; i += nt) box[i] = 0.f;
+            for (int q = 0; q < zn; ++q)
+                for (int i = threadIdx.x; i < nbox; i += nt) box[q * bstride + i] = 0.f;
             __syncthreads();
             const int ncol = t.cr * t.cc;
             for (int col = 0; col < ncol; ++col) {
@@ -2181,53 +2188,71 @@ __global__ void __launch_bounds__(kGatherThreads) k_scatter_col(Scatter g, const
                     const int p = g.perm[t.pbase + i];
                     const int r = p >> t.lnc, cidx = p & (t.nc - 1);
                     const float2 c = cb[(size_t)(t.r0 + r + g.c_ro) * g.c_row + t.c0 + cidx + g.c_co];
-                    float v = V[(size_t)(t0.r0 + r) * g.v_row + t0.c0 + cidx];
-                    if (g.scale) v *= g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx];
+                    const size_t vo = (size_t)(t0.r0 + r) * g.v_row + t0.c0 + cidx;
+                    const float sc = g.scale ? g.scale[(size_t)(t0.r0 + r) * g.s_row + t0.c0 + cidx] : 1.f;
+                    float v[KZ];
+#pragma unroll
+                    for (int q = 0; q < KZ; ++q) v[q] = q < zn ? V[vo + (size_t)q * g.v_z] : 0.f;
+                    if (g.scale) {
+#pragma unroll
+                        for (int q = 0; q < KZ; ++q) v[q] *= sc;
+                    }
                     const float fx = floorf(c.x), fy = floorf(c.y);
                     const float ax = c.x - fx, ay = c.y - fy;
                     const int lx = min(max((int)fx - 1 - t.x0, 0), t.bw - 4);
                     const int ly = min(max((int)fy - 1 - t.y0, 0), t.bh - 4);
-                    float* q = box + ly * bs + lx;
                     if (g.cubic) {
                         float wx[4], wy[4];
                         bspline_w(ax, wx);
                         bspline_w(ay, wy);
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const float vb = wy[b] * v;
-                            float* qr = q + b * bs;
-                            const float s0 = qr[0], s1 = qr[1], s2 = qr[2], s3 = qr[3];
-                            qr[0] = s0 + wx[0] * vb;
-                            qr[1] = s1 + wx[1] * vb;
-                            qr[2] = s2 + wx[2] * vb;
-                            qr[3] = s3 + wx[3] * vb;
+                        for (int q = 0; q < KZ; ++q) {
+                            if (q >= zn) break;
+                            float* qb = box + q * bstride + ly * bs + lx;
+#pragma unroll
+                            for (int b = 0; b < 4; ++b) {
+                                const float vb = wy[b] * v[q];
+                                float* qr = qb + b * bs;
+                                const float s0 = qr[0], s1 = qr[1], s2 = qr[2], s3 = qr[3];
+                                qr[0] = s0 + wx[0] * vb;
+                                qr[1] = s1 + wx[1] * vb;
+                                qr[2] = s2 + wx[2] * vb;
+                                qr[3] = s3 + wx[3] * vb;
+                            }
                         }
                     } else {
-                        float* q1 = q + bs + 1;
-                        float* q2 = q1 + bs;
-                        q1[0] += (1.f - ay) * (1.f - ax) * v;
-                        q1[1] += (1.f - ay) * ax * v;
-                        q2[0] += ay * (1.f - ax) * v;
-                        q2[1] += ay * ax * v;
+#pragma unroll
+                        for (int q = 0; q < KZ; ++q) {
+                            if (q >= zn) break;
+                            float* q1 = box + q * bstride + ly * bs + lx + bs + 1;
+                            float* q2 = q1 + bs;
+                            q1[0] += (1.f - ay) * (1.f - ax) * v[q];
+                            q1[1] += (1.f - ay) * ax * v[q];
+                            q2[0] += ay * (1.f - ax) * v[q];
+                            q2[1] += ay * ax * v[q];
+                        }
                     }
                 }
                 __syncthreads();
             }
             const float inv = 1.0f / (float)t.bw;
             const int nb = t.bw * t.bh;
-            if (g.stage) {
-                float* B = g.stage + (size_t)z * g.stage_z + g.box_off[(size_t)tile * g.group + gi];
-                for (int i = threadIdx.x; i < nb; i += nt) {
-                    const int yy = (int)(((float)i + 0.5f) * inv);
-                    B[i] = box[yy * bs + (i - yy * t.bw)];
-                }
-            } else {
-                float* D = g.dst + (size_t)z * g.dst_z + (size_t)t.span * g.dst_span;
-                for (int i = threadIdx.x; i < nb; i += nt) {
-                    const int yy = (int)(((float)i + 0.5f) * inv);
-                    const int xx = i - yy * t.bw;
-                    const float v = box[yy * bs + xx];
-                    if (v != 0.f) unsafeAtomicAdd(D + (size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W), v);
+            for (int q = 0; q < zn; ++q) {
+                const float* bq = box + q * bstride;
+                if (g.stage) {
+                    float* B = g.stage + (size_t)(zb + q) * g.stage_z + g.box_off[(size_t)tile * g.group + gi];
+                    for (int i = threadIdx.x; i < nb; i += nt) {
+                        const int yy = (int)(((float)i + 0.5f) * inv);
+                        B[i] = bq[yy * bs + (i - yy * t.bw)];
+                    }
+                } else {
+                    float* D = g.dst + (size_t)(zb + q) * g.dst_z + (size_t)t.span * g.dst_span;
+                    for (int i = threadIdx.x; i < nb; i += nt) {
+                        const int yy = (int)(((float)i + 0.5f) * inv);
+                        const int xx = i - yy * t.bw;
+                        const float v = bq[yy * bs + xx];
+                        if (v != 0.f) unsafeAtomicAdd(D + (size_t)wrapi(t.y0 + yy, g.H) * g.W + wrapi(t.x0 + xx, g.W), v);
+                    }
                 }
             }
             __syncthreads();
@@ -2248,11 +2273,12 @@ void launch_scatter(const Scatter& g0, const GTile* tiles, int ntiles, int kind,
     dim3 grid(ng * ((g.nz + g.zgroup - 1) / g.zgroup));
     size_t lds = (size_t)g.box_floats * sizeof(float);
     if (g.perm) {
+        const size_t ldsc = lds * kScatterSlices;
         switch (kind) {
-            case 0: hipLaunchKernelGGL(k_scatter_col<0>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-            case 1: hipLaunchKernelGGL(k_scatter_col<1>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-            case 2: hipLaunchKernelGGL(k_scatter_col<2>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
-            default: hipLaunchKernelGGL(k_scatter_col<3>, grid, dim3(kGatherThreads), lds, s, g, tiles); break;
+            case 0: hipLaunchKernelGGL(k_scatter_col<0>, grid, dim3(kGatherThreads), ldsc, s, g, tiles); break;
+            case 1: hipLaunchKernelGGL(k_scatter_col<1>, grid, dim3(kGatherThreads), ldsc, s, g, tiles); break;
+            case 2: hipLaunchKernelGGL(k_scatter_col<2>, grid, dim3(kGatherThreads), ldsc, s, g, tiles); break;
+            default: hipLaunchKernelGGL(k_scatter_col<3>, grid, dim3(kGatherThreads), ldsc, s, g, tiles); break;
         }
     }
     switch (kind) {

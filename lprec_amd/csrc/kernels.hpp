@@ -142,6 +142,10 @@ struct Gather {
 };
 constexpr int kGatherThreads = 256;
 constexpr int kGatherPPT = 8;   // points per thread: tiles hold <= 2048 points
+// slices per colour pass of the coloured scatter (k_scatter_col); 2 measured
+// no faster (0.692 against 0.699 ms per call, N0 = 1024, 16 slices) for
+// twice the LDS
+constexpr int kScatterSlices = 1;
 #ifndef LP_GATHER_PIPE
 #define LP_GATHER_PIPE 2
 #endif
