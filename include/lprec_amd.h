@@ -140,7 +140,11 @@ int lp_info(const lp_handle* h, int64_t* info);
  * odd-radix work items of 4 output pairs instead of 8; default 14, the
  * fastest measured at N = 2064; recompiles), key 13 = slices per block of
  * lp_fwd's log-polar -> polar gather (>= 1, default 4; each block reads its
- * tile's coordinates once).  Returns LP_ERR_ARG for bad values. */
+ * tile's coordinates once), key 14 = lp_fwd's image -> log-polar gather
+ * reads every tap from global memory, one thread per point (1), instead of
+ * from LDS-staged tile boxes (0, the default): a measurement of the sampling
+ * that a gather fused into the row pass would do.  Returns LP_ERR_ARG for bad
+ * values. */
 int lp_set_tuning(lp_handle* h, int key, int value);
 int lp_get_tuning(const lp_handle* h, int key, int* value);
 

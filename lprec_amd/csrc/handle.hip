@@ -78,6 +78,9 @@ struct lp_handle {
     // 2028 MB: the blocks of neighbouring tiles, which share G rows, run
     // further apart), so the default stays at 4
     int pzgroup = 4;
+    // image -> log-polar gather from global memory, no LDS tiles (measurement
+    // of the sampling a fused gather + row pass would do; off)
+    int gather_direct = 0;
     // row / column passes compiled at run time for lengths the radix-16
     // kernels do not cover (rtc_build): setting, module, functions (rows
     // fwd, rows inv, cols) and their launch shapes
@@ -1386,7 +1389,8 @@ static int fwd_chain(lp_handle* h, float* R, const float* f, int ns0, hipStream_
         g.scale = h->d_erho; g.s_row = h->Nt;
         g.out = w.G; g.o_z = grid; g.o_span = (long long)ns0 * grid; g.o_row = h->Nt;
         g.nz = ns0; g.zgroup = h->gzgroup; g.group = 1; g.cubic = h->cubic; g.box_floats = h->boxF;
-        lpk::launch_gather(g, h->d_tilesF, h->ntilesF, 0, s);
+        if (h->gather_direct) lpk::launch_gather_direct(g, h->d_rangeF, h->Nr, h->Nt, s);
+        else lpk::launch_gather(g, h->d_tilesF, h->ntilesF, 0, s);
     }
     mark(3);
     if (h->stop_after == 3) return check_launch(h, "lp_fwd launch");
@@ -1742,6 +1746,10 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
                 rtc_build(h);
             }
             return LP_OK;
+        case 14:
+            if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "direct gather must be 0 or 1");
+            h->gather_direct = value;
+            return LP_OK;
         case 13:
             if (value < 1) return h->fail(LP_ERR_ARG, "slices per polar gather block must be >= 1");
             h->pzgroup = value;
@@ -1845,6 +1853,7 @@ int lp_get_tuning(const lp_handle* h, int key, int* value) {
         case 11: *value = h->rtc_cols_nt; return LP_OK;
         case 12: *value = h->rtc_cols_var; return LP_OK;
         case 13: *value = h->pzgroup; return LP_OK;
+        case 14: *value = h->gather_direct; return LP_OK;
     }
     return LP_ERR_ARG;
 }

@@ -694,6 +694,30 @@ __global__ void __launch_bounds__(kGatherThreads, 5) k_gather1(Gather g, const G
     }
 }
 
+// Image -> log-polar samples taken straight from the prefiltered image in
+// global memory (16 taps per point through the caches), one thread per point
+// and no LDS box: the sampling that a row pass fused with this gather would
+// have to do (its rows' source arcs do not fit in LDS).  Kept to measure that
+// design against the LDS-tiled k_gather1<0>; only the sampled theta range of
+// each row is written, as by the tiles.  Grid (Ntheta / 256, Nrho, 3 nz).
+__global__ void __launch_bounds__(256) k_gather_direct(Gather g, const int2* __restrict__ range, int Nr, int Nt) {
+    const int th = blockIdx.x * 256 + threadIdx.x;
+    const int row = blockIdx.y;
+    const int span = blockIdx.z / g.nz, z = blockIdx.z - span * g.nz;
+    if (th >= Nt) return;
+    const int2 rg = range ? range[(size_t)span * Nr + row] : make_int2(0, Nt);
+    if (th < rg.x || th >= rg.y) return;
+    const float2 c = g.coord[(size_t)span * g.c_span + (size_t)row * g.c_row + th];
+    float v = 0.f;
+    if (c.x == c.x) v = sample(g.src + (size_t)z * g.src_z + (size_t)span * g.src_span, g.W, g.H, c.x, c.y, g.cubic != 0);
+    if (g.scale) v *= g.scale[(size_t)row * g.s_row + th];
+    g.out[(size_t)z * g.o_z + (size_t)span * g.o_span + (size_t)row * g.o_row + th] = v;
+}
+
+void launch_gather_direct(const Gather& g, const int2* range, int Nr, int Nt, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_direct, dim3((Nt + 255) / 256, Nr, 3 * g.nz), dim3(256), 0, s, g, range, Nr, Nt);
+}
+
 template <int KIND, int GROUP>
 __global__ void __launch_bounds__(kGatherThreads) k_gather(Gather g, const GTile* __restrict__ tiles) {
     static_assert(GROUP > 1, "single-span gathers run k_gather1");

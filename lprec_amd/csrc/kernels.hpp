@@ -152,6 +152,10 @@ constexpr int kScatterSlices = 1;
 constexpr int kGatherPipe = LP_GATHER_PIPE;   // points whose coordinates are in flight ahead of the sampling
 // kind: 0 image->LP, 1 sinogram->LP, 2 LP->polar, 3 LP->Cartesian (names the kernel instance)
 void launch_gather(const Gather& g, const GTile* tiles, int ntiles, int kind, hipStream_t s);
+// The image -> log-polar gather with every tap read from global memory, one
+// thread per point, only the theta range [3][Nrho] of each row (a measurement
+// of the sampling a row pass fused with the gather would do)
+void launch_gather_direct(const Gather& g, const int2* range, int Nr, int Nt, hipStream_t s);
 
 // ---- transposes (exact adjoints of the stages above) -----------------------
 // Transpose of a Gather: the value at every output point of every tile (times

@@ -149,3 +149,30 @@ def test_polar_gather_slices_per_block_keep_results(torch_cuda):
     finally:
         h.set_tuning(13, 4)
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.gpu
+def test_direct_image_gather_matches_tiled(torch_cuda):
+    """The image -> log-polar gather read from global memory (tuning key 14,
+    a measurement path) gives the tiled gather's fwd to rounding (1e-6
+    relative: the B-spline weights are formed in a different order)."""
+    torch = torch_cuda
+    N0, P0, ns = 256, 300, 3
+    lp = lpTransform(N0, P0, ns, "none", N0 / 2, "cubic")
+    lp.precompute(1)
+    lp.initcmem(1, 0)
+    h = lp.clphandle[0]
+    f = torch.from_numpy(phantom_stack(ns, N0, 12, seed=6)).cuda()
+    outs = []
+    try:
+        for d in (0, 1):
+            h.set_tuning(14, d)
+            R = torch.zeros((ns, P0, N0), device="cuda")
+            lp.fwdp(R, f, 0)
+            torch.cuda.synchronize()
+            outs.append(R.cpu().numpy())
+    finally:
+        h.set_tuning(14, 0)
+    e = rel(outs[1], outs[0])
+    print("direct vs tiled image gather: %.2e" % e, flush=True)
+    assert e <= 1e-6
