@@ -460,7 +460,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-kernel HBM ceiling")
-    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (rows, zgroup, colchunk, gzgroup, fft16, boxcap, fuse, boxstride, rtc, rtccols, rtcvar, pzgroup, direct)")
+    ap.add_argument("--tune", default="", help="schedule knobs key=value,... (the keys of lprec_amd.tune.KNOBS)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--pmc-summary", default="", metavar="DIR",
@@ -523,8 +523,7 @@ def main():
         return
     N0, P0, ns = args.N, args.nproj, args.slices
     cor = N0 / 2
-    knobs = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6,
-             "boxstride": 9, "rtc": 10, "rtccols": 11, "rtcvar": 12, "pzgroup": 13, "direct": 14}
+    from lprec_amd.tune import KNOBS as knobs
     sched = {}
     for kv in filter(None, args.tune.split(",")):
         k, v = kv.split("=")

@@ -17,8 +17,8 @@ import torch
 
 from ._lib import LprecError
 
-KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6, "boxstride": 9,
-         "rtc": 10, "rtccols": 11, "rtcvar": 12, "pzgroup": 13, "direct": 14}
+KNOBS = {"rows": 0, "zgroup": 1, "colchunk": 2, "gzgroup": 3, "fft16": 4, "boxcap": 5, "fuse": 6, "polart": 7,
+         "staged": 8, "boxstride": 9, "rtc": 10, "rtccols": 11, "rtcvar": 12, "pzgroup": 13, "direct": 14}
 # knobs that can change after the tables are built (boxcap and boxstride are fixed at init)
 RUNTIME_KNOBS = ("rows", "zgroup", "colchunk", "gzgroup", "fft16", "fuse", "pzgroup")
 SPACE = {
