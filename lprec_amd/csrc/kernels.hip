@@ -2427,9 +2427,14 @@ void launch_polar_transpose(const PolarT& p, hipStream_t s) {
 }
 
 // One block per destination tile of 32 x 32 points (4 per thread); the
-// tile's box list is walked once per 4 slices.
+// tile's box list is walked once per 4 slices, its rectangles and offsets
+// staged in LDS 128 at a time (one burst of list reads instead of a
+// dependent pair of global loads per box), and two boxes' value loads are in
+// flight together.  Every point adds its boxes in list order.
 __global__ void __launch_bounds__(256) k_box_sum(BoxSum b) {
-    constexpr int ZC = 4;
+    constexpr int ZC = 4, LB = 128;
+    __shared__ int4 srect[LB];
+    __shared__ int soff[LB];
     const int t = blockIdx.x;
     const int plane = t / (b.tiles_x * b.tiles_y);
     const int tt = t - plane * b.tiles_x * b.tiles_y;
@@ -2445,23 +2450,33 @@ __global__ void __launch_bounds__(256) k_box_sum(BoxSum b) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) acc[q][e] = 0.f;
         const float* St = b.stage + (size_t)z0 * b.stage_z;
-        for (int i = beg; i < end; ++i) {
-            const int id = b.tile_box[i];
-            const int4 r = b.box_rect[id];
-            const float* B = St + b.box_off[id];
-            // box origins lie within one period of the plane: one wrap at most
-            int dx = X - r.x;
-            dx += dx < 0 ? b.W : (dx >= b.W ? -b.W : 0);
-            if (dx >= r.z) continue;
+        for (int c0 = beg; c0 < end; c0 += LB) {
+            const int nb = min(LB, end - c0);
+            __syncthreads();   // the previous round's entries are read
+            if ((int)threadIdx.x < nb) {
+                const int id = b.tile_box[c0 + threadIdx.x];
+                srect[threadIdx.x] = b.box_rect[id];
+                soff[threadIdx.x] = b.box_off[id];
+            }
+            __syncthreads();
+#pragma unroll 2
+            for (int i = 0; i < nb; ++i) {
+                const int4 r = srect[i];
+                const float* B = St + soff[i];
+                // box origins lie within one period of the plane: one wrap at most
+                int dx = X - r.x;
+                dx += dx < 0 ? b.W : (dx >= b.W ? -b.W : 0);
+                if (dx >= r.z) continue;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                int dy = Y0 + 8 * e - r.y;
-                dy += dy < 0 ? b.H : (dy >= b.H ? -b.H : 0);
-                if (dy < r.w) {
-                    const float* Bq = B + dy * r.z + dx;
+                for (int e = 0; e < 4; ++e) {
+                    int dy = Y0 + 8 * e - r.y;
+                    dy += dy < 0 ? b.H : (dy >= b.H ? -b.H : 0);
+                    if (dy < r.w) {
+                        const float* Bq = B + dy * r.z + dx;
 #pragma unroll
-                    for (int q = 0; q < ZC; ++q)
-                        if (q < zn) acc[q][e] += Bq[(size_t)q * b.stage_z];
+                        for (int q = 0; q < ZC; ++q)
+                            if (q < zn) acc[q][e] += Bq[(size_t)q * b.stage_z];
+                    }
                 }
             }
         }
