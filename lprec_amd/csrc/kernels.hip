@@ -2427,12 +2427,12 @@ void launch_polar_transpose(const PolarT& p, hipStream_t s) {
 }
 
 // One block per destination tile of 32 x 32 points (4 per thread); the
-// tile's box list is walked once per 4 slices, its rectangles and offsets
+// tile's box list is walked once per 8 slices, its rectangles and offsets
 // staged in LDS 128 at a time (one burst of list reads instead of a
 // dependent pair of global loads per box), and two boxes' value loads are in
 // flight together.  Every point adds its boxes in list order.
 __global__ void __launch_bounds__(256) k_box_sum(BoxSum b) {
-    constexpr int ZC = 4, LB = 128;
+    constexpr int ZC = 8, LB = 128;
     __shared__ int4 srect[LB];
     __shared__ int soff[LB];
     const int t = blockIdx.x;
