@@ -91,8 +91,8 @@ struct lp_handle {
     int rtc_cols_var = 14;
     bool rtc_tried = false;
     hipModule_t rtc_mod = nullptr;
-    void* rtc_fn[3] = {nullptr, nullptr, nullptr};
-    int rtc_R = 0, rtc_rows_nt = 0, rtc_cols_nt = 0;
+    void* rtc_fn[4] = {nullptr, nullptr, nullptr, nullptr};   // + the filtered sinogram pair pass
+    int rtc_R = 0, rtc_rows_nt = 0, rtc_cols_nt = 0, rtc_pair_nt = 0;
     size_t lds_limit = 65536;
     std::string err;
     std::vector<void*> allocs;
@@ -857,10 +857,12 @@ void rtc_build(lp_handle* h) {
     if (h->rtc_mod) {
         (void)hipModuleUnload(h->rtc_mod);
         h->rtc_mod = nullptr;
-        h->rtc_fn[0] = h->rtc_fn[1] = h->rtc_fn[2] = nullptr;
+        h->rtc_fn[0] = h->rtc_fn[1] = h->rtc_fn[2] = h->rtc_fn[3] = nullptr;
     }
     const bool rows = !pow2_in(h->h, 256, 4096), cols = !pow2_in(h->Nr, 256, 4096);
-    if (!rows && !cols) return;
+    // the filtered pair pass has no register kernel at any length
+    const bool pair = h->has_filter;
+    if (!rows && !cols && !pair) return;
     const char* v = std::getenv("LPREC_AMD_VERBOSE");
     const bool verbose = v && v[0] == '1';
     hipDeviceProp_t prop;
@@ -879,6 +881,10 @@ void rtc_build(lp_handle* h) {
             std::to_string(rnt) + ", " + wpe + ", 1" + hr + ">",
         "lpct::k_cols_ct<" + std::to_string(h->Nr) + ", " + std::to_string(cnt) + ", " + wpe + ", " +
             std::to_string(h->rtc_cols_var & 3) + mr + ">"};
+    const int pnt = rtc_threads(2LL * h->N);
+    if (pair)
+        names.push_back("lpct::k_sino_rows_pair_ct<" + std::to_string(h->N) + ", " + std::to_string(pnt) + ", " + wpe +
+                        radix_list(h->plan_2N) + ">");
     std::string err;
     const int pp = (h->rtc_cols_var & 8) ? 4 : 8;
     auto code = rtc_compile(arch, pp, names, err);
@@ -888,8 +894,8 @@ void rtc_build(lp_handle* h) {
         err = "hipModuleLoadData failed";
         mod = nullptr;
     }
-    void* fn[3] = {nullptr, nullptr, nullptr};
-    for (int i = 0; mod && i < 3; ++i) {
+    void* fn[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int i = 0; mod && i < (int)names.size(); ++i) {
         hipFunction_t f = nullptr;
         if (hipModuleGetFunction(&f, mod, code->lowered[i].c_str()) != hipSuccess) {
             (void)hipGetLastError();
@@ -909,12 +915,14 @@ void rtc_build(lp_handle* h) {
     h->rtc_fn[0] = rows ? fn[0] : nullptr;
     h->rtc_fn[1] = rows ? fn[1] : nullptr;
     h->rtc_fn[2] = cols ? fn[2] : nullptr;
+    h->rtc_fn[3] = pair ? fn[3] : nullptr;
     h->rtc_R = R;
     h->rtc_rows_nt = rnt;
     h->rtc_cols_nt = cnt;
+    h->rtc_pair_nt = pnt;
     if (verbose)
-        std::fprintf(stderr, "lprec_amd: run-time compiled passes: %s | %s | %s\n", names[0].c_str(), names[1].c_str(),
-                     names[2].c_str());
+        for (size_t i = 0; i < names.size(); ++i)
+            if (h->rtc_fn[i]) std::fprintf(stderr, "lprec_amd: run-time compiled pass: %s\n", names[i].c_str());
 }
 
 }  // namespace
@@ -1288,6 +1296,7 @@ int lp_init_adj(lp_handle* h, const int32_t* pi, const float* pf) {
         h->has_filter = true;
     }
     h->has_adj = true;
+    if (h->has_filter) h->rtc_tried = false;   // (re)build with the filtered pair pass
     rtc_build(h);
     return LP_OK;
 }
@@ -1482,6 +1491,8 @@ int lp_adj(lp_handle* h, float* f, const float* R, int ns0, void* stream) {
     sp.tw2N = h->has_filter ? h->tw[2 * h->Lf] : nullptr;
     sp.plan2N = h->plan_2N;
     sp.tw2Nc = h->has_filter ? h->tw[2 * N] : nullptr;
+    sp.rtc_pair = h->rtc_mode && h->rtc_mod ? h->rtc_fn[3] : nullptr;
+    sp.rtc_pair_nt = h->rtc_pair_nt;
     hipEvent_t* ev = h->prof_begin(1);
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     mark(0);
@@ -1646,6 +1657,8 @@ int lp_adj_T(lp_handle* h, float* R, const float* f, int ns0, void* stream) {
     sp.tw2N = h->has_filter ? h->tw[2 * h->Lf] : nullptr;
     sp.plan2N = h->plan_2N;
     sp.tw2Nc = h->has_filter ? h->tw[2 * N] : nullptr;
+    sp.rtc_pair = nullptr;
+    sp.rtc_pair_nt = 0;
     if (h->cubic) {
         lpk::launch_col_iir_T(acc, h->d_tmp, ns0, h->P, N, h->colchunk, s);
         lpk::launch_sino_rows_T(h->d_tmp, R, ns0, sp, true, s);

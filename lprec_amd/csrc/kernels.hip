@@ -112,52 +112,6 @@ __device__ __forceinline__ void stage_in(int n, Load load, Store store) {
     }
 }
 
-// Cubic B-spline prefilter of one line held in LDS (ConvertToInterpolationCoefficients,
-// main_kernels.cuh:97-142).  Each thread owns a chunk and restarts the causal and
-// anticausal recursions kHalo samples outside it (truncation |z|^17 < 2e-10), except
-// at the line ends where the reference's own initialisations are used.
-// (tid, nt: this thread's index among the nt threads that share the line;
-// every thread of the block calls it, since it synchronises the block)
-__device__ void iir_line_lds(float* line, float* tmp, int L, int tid, int nt) {
-    const float z = kPole;
-    // odd chunk length: lanes start T floats apart, so an odd T keeps the
-    // lanes of a wave on distinct banks
-    const int T = ((L + nt - 1) / nt) | 1;
-    const int s = tid * T;
-    const int e = min(s + T, L);
-    if (s < L) {
-        float c;
-        int start;
-        if (s - kHalo <= 0) {
-            int hz = min(kHorizon, L);
-            float sum = line[0];
-            float zn = z;
-            for (int n = 0; n < hz; ++n) { sum += zn * line[n]; zn *= z; }
-            c = kLambda * sum;
-            start = 0;
-        } else {
-            start = s - kHalo;
-            c = kLambda * line[start];
-        }
-        if (start >= s) tmp[start] = c;
-        for (int n = start + 1; n < e; ++n) {
-            c = kLambda * line[n] + z * c;
-            if (n >= s) tmp[n] = c;
-        }
-    }
-    __syncthreads();
-    if (s < L) {
-        int end = min(e - 1 + kHalo, L - 1);
-        float c = (end == L - 1) ? (z / (z - 1.0f)) * tmp[L - 1] : -z * tmp[end];
-        if (end < e) line[end] = c;
-        for (int n = end - 1; n >= s; --n) {
-            c = z * (c - tmp[n]);
-            if (n < e) line[n] = c;
-        }
-    }
-    __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
 // fwd input: copy3Dshifted (lpRgpu.cu:189) + x-direction prefilter (lpRgpu.cu:219-240)
 __global__ void k_image_rows(const float* __restrict__ f, float* __restrict__ fc, int N0, int N, int o, int cubic) {
@@ -1642,6 +1596,15 @@ void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, boo
         return;
     }
     // two kept angles per block, one complex FFT pair of length 2N
+    if (sp.rtc_pair) {
+        SinoPrep q = sp;
+        int cb = (int)cubic;
+        void* args[] = {&R, &Rc, &q, &cb};
+        if (hipModuleLaunchKernel((hipFunction_t)q.rtc_pair, (sp.P0 + 1) / 2, nz, 1, q.rtc_pair_nt, 1, 1, 0, s, args,
+                                  nullptr) == hipSuccess)
+            return;
+        (void)hipGetLastError();   // the generic kernel below does the pass
+    }
     int nt = threads_for(2LL * sp.N);
     int ept = ept_for(2LL * sp.N, nt);
     size_t lds = (size_t)lds_slots(2 * sp.N) * sizeof(float2) + (size_t)(4 * sp.N) * sizeof(float);

@@ -18,6 +18,52 @@ constexpr float kLambda = 6.0f;                   // (1 - z)(1 - 1/z)
 constexpr int kHorizon = 12;                      // causal init horizon (main_kernels.cuh:99)
 constexpr int kHalo = 16;                         // |z|^17 < 2e-10: chunk halo of the IIR sweeps
 
+// Cubic B-spline prefilter of one line held in LDS (ConvertToInterpolationCoefficients,
+// main_kernels.cuh:97-142).  Each thread owns a chunk and restarts the causal and
+// anticausal recursions kHalo samples outside it (truncation |z|^17 < 2e-10), except
+// at the line ends where the reference's own initialisations are used.
+// (tid, nt: this thread's index among the nt threads that share the line;
+// every thread of the block calls it, since it synchronises the block)
+__device__ inline void iir_line_lds(float* line, float* tmp, int L, int tid, int nt) {
+    const float z = kPole;
+    // odd chunk length: lanes start T floats apart, so an odd T keeps the
+    // lanes of a wave on distinct banks
+    const int T = ((L + nt - 1) / nt) | 1;
+    const int s = tid * T;
+    const int e = min(s + T, L);
+    if (s < L) {
+        float c;
+        int start;
+        if (s - kHalo <= 0) {
+            int hz = min(kHorizon, L);
+            float sum = line[0];
+            float zn = z;
+            for (int n = 0; n < hz; ++n) { sum += zn * line[n]; zn *= z; }
+            c = kLambda * sum;
+            start = 0;
+        } else {
+            start = s - kHalo;
+            c = kLambda * line[start];
+        }
+        if (start >= s) tmp[start] = c;
+        for (int n = start + 1; n < e; ++n) {
+            c = kLambda * line[n] + z * c;
+            if (n >= s) tmp[n] = c;
+        }
+    }
+    __syncthreads();
+    if (s < L) {
+        int end = min(e - 1 + kHalo, L - 1);
+        float c = (end == L - 1) ? (z / (z - 1.0f)) * tmp[L - 1] : -z * tmp[end];
+        if (end < e) line[end] = c;
+        for (int n = end - 1; n >= s; --n) {
+            c = z * (c - tmp[n]);
+            if (n < e) line[n] = c;
+        }
+    }
+    __syncthreads();
+}
+
 struct Src {            // a stack of 2-D float grids read through bicubic / bilinear taps
     const float* p;     // [nz][H][W]
     int W, H;
@@ -38,6 +84,10 @@ struct SinoPrep {
     const float2* tw2N;       // W_{2Lh}^k, k <= Lh/2
     FftPlan plan2N;           // complex FFT of length 2N (two filtered lines packed as x0 + i x1)
     const float2* tw2Nc;      // W_{2N}^j, j < 2N
+    // the filtered pair pass compiled at run time for this N (hipFunction_t of
+    // rtc_kernels.hpp, rtc_pair_nt threads; null: the generic kernel)
+    void* rtc_pair;
+    int rtc_pair_nt;
 };
 void launch_sino_rows(const float* R, float* Rc, int nz, const SinoPrep& sp, bool cubic, hipStream_t s);
 // y-direction prefilter of [nz][L][W] (columns), chunked with halo, in -> out

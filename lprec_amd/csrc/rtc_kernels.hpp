@@ -224,4 +224,70 @@ __global__ void __launch_bounds__(NT, WPE) k_rows_inv_ct(lpk::RowPass rp, float*
     }
 }
 
+// Filtered sinogram rows (k_sino_rows_pair): two kept angles per block, the
+// FBP filter as one complex FFT pair of length M = 2N of x0 + i x1 (the
+// filter spectrum is real and even), then both lines' prefilters side by
+// side.  The rows are read straight into the FFT buffer; the buffer the
+// inverse FFT does not end in holds the two lines and the prefilter's
+// scratch.  Grid ((P0 + 1) / 2, nz), NT threads.
+template <int N, int NT, int WPE, int... Rs>
+__global__ void __launch_bounds__(NT, WPE) k_sino_rows_pair_ct(const float* __restrict__ R, float* __restrict__ Rc,
+                                                               lpk::SinoPrep sp, int cubic) {
+    constexpr int M = 2 * N;
+    __shared__ float2 sa[lds_slots(M)], sb[lds_slots(M)];
+    __shared__ __align__(16) float2 wq[odd_slots<Rs...>()];
+    OddTables<Rs...>::template fill<M>(wq, sp.tw2Nc, NT);
+    const int a0 = 2 * blockIdx.x, z = blockIdx.y;
+    const int nrow = min(2, sp.P0 - a0);
+    for (int u = 0; u < (cubic ? 0 : nrow); ++u)
+        for (int j = 1; j < sp.os; ++j) {
+            float* dz = Rc + ((size_t)z * sp.P + (size_t)(a0 + u) * sp.os + j) * N;
+            for (int x = threadIdx.x; x < N; x += NT) dz[x] = 0.f;
+        }
+    {
+        const float* src0 = R + ((size_t)z * sp.P0 + a0) * sp.N0;
+        const float* src1 = R + ((size_t)z * sp.P0 + min(a0 + 1, sp.P0 - 1)) * sp.N0;
+        for (int x0 = threadIdx.x; x0 < M; x0 += 4 * NT) {
+            float v0[4], v1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int xs = min(max(x0 + u * NT - sp.lb, 0), sp.N0 - 1);
+                v0[u] = src0[xs];
+                v1[u] = src1[xs];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int x = x0 + u * NT;
+                if (x < M) sa[lidx(x)] = x < N ? make_float2(v0[u], nrow > 1 ? v1[u] : 0.f) : make_float2(0.f, 0.f);
+            }
+        }
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    float2* xp = fft<M, 1, M, NT>(sa, sb, sp.tw2Nc, wq, -1, Radices<Rs...>{}, tid);
+    for (int k = tid; k < M; k += NT) {
+        const float hk = sp.H2[k <= N ? k : M - k];
+        const float2 v = xp[lidx(k)];
+        xp[lidx(k)] = make_float2(v.x * hk, v.y * hk);
+    }
+    __syncthreads();
+    float2* yp = fft<M, 1, M, NT>(xp, xp == sa ? sb : sa, sp.tw2Nc, wq, +1, Radices<Rs...>{}, tid);
+    float* line = reinterpret_cast<float*>(yp == sa ? sb : sa);   // [2][N] lines, then [2][N] scratch
+    float* tmp = line + 2 * N;
+    for (int e = tid; e < N; e += NT) {
+        const float2 v = yp[lidx(e)];
+        line[e] = v.x;
+        line[N + e] = v.y;
+    }
+    __syncthreads();
+    // the two lines' prefilters run side by side on the two halves of the block
+    constexpr int half = NT / 2;
+    const int u = tid < half ? 0 : 1;
+    if (cubic) lpk::iir_line_lds(line + u * N, tmp + u * N, N, tid - u * half, half);
+    for (int v = 0; v < nrow; ++v) {
+        float* dst = Rc + ((size_t)z * sp.P + (size_t)(a0 + v) * sp.os) * N;
+        for (int x = tid; x < N; x += NT) dst[x] = line[v * N + x];
+    }
+}
+
 }  // namespace lpct

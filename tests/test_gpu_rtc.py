@@ -1,6 +1,7 @@
 """Row and column passes compiled at run time (hipRTC) for transform lengths
-with odd factors, against the generic mixed-radix kernels they replace
-(tuning key 10 = 0) and against the oracle.
+with odd factors, and the filtered sinogram pair pass at every length,
+against the generic kernels they replace (tuning key 10 = 0) and against the
+oracle.
 
 The compiled passes run the same Stockham stages with the same butterflies
 and twiddle products; only the index arithmetic is resolved at compile time,
@@ -27,6 +28,7 @@ CASES = [
     (2048, 300, 2, "ramp", 1024.912, "cubic"), # N = 2064: h = 1032 = 2^3 3 43, 4128 = 2^5 3 43
     (250, 300, 2, "hamming", 127.3, "linear"), # N = 256: h = 128 is below the radix-16 row kernels' range
     (1024, 360, 2, "none", 512, "cubic"),      # N = 1024: the radix-16 kernels, no run-time passes
+    (1024, 360, 2, "shepp-logan", 512, "cubic"),  # N = 1024 with a filter: the compiled sinogram pair pass only
 ]
 
 
@@ -56,7 +58,8 @@ def test_runtime_compiled_passes_match_generic_kernels(torch_cuda, N0, P0, ns, f
     lp.initcmem(1, 0)
     h = lp.clphandle[0]
     N = h.info()["N"]
-    expect = 0 if _covered(N // 2) and _covered(2 * N) else 1
+    # the filtered sinogram pair pass is compiled at every length
+    expect = 0 if _covered(N // 2) and _covered(2 * N) and filt.lower() == "none" else 1
     assert h.get_tuning(10) == expect, "run-time compiled passes loaded: %d, expected %d" % (h.get_tuning(10), expect)
     a = _run(lp, f, R, torch)
     h.set_tuning(10, 0)
