@@ -869,22 +869,38 @@ void rtc_build(lp_handle* h) {
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) return;
     std::string arch(prop.gcnArchName);
     arch = arch.substr(0, arch.find(':'));
-    const int R = std::min(h->R, 8), ldm = row_ldm(h, R);
+    // rows per block of the compiled row passes: at most 8 (the generic
+    // kernels' limit) and few enough that their two LDS buffers fit
+    int R = std::min(h->R, 8);
+    while (R > 1 && 2 * (size_t)lds_slots(R * row_ldm(h, R)) * sizeof(float2) > h->lds_limit) R /= 2;
+    const int ldm = row_ldm(h, R);
     const int rnt = rtc_threads((long long)R * h->h);
     const int cnt = h->rtc_cols_req ? h->rtc_cols_req : rtc_threads(h->Nr);
     const std::string hr = radix_list(h->plan_h), mr = radix_list(h->plan_M);
     const std::string wpe = (h->rtc_cols_var & 4) ? "4" : "2";
-    std::vector<std::string> names = {
-        "lpct::k_rows_fwd_ct<" + std::to_string(h->h) + ", " + std::to_string(R) + ", " + std::to_string(ldm) + ", " +
-            std::to_string(rnt) + ", " + wpe + ", 1" + hr + ">",
-        "lpct::k_rows_inv_ct<" + std::to_string(h->h) + ", " + std::to_string(R) + ", " + std::to_string(ldm) + ", " +
-            std::to_string(rnt) + ", " + wpe + ", 1" + hr + ">",
-        "lpct::k_cols_ct<" + std::to_string(h->Nr) + ", " + std::to_string(cnt) + ", " + wpe + ", " +
-            std::to_string(h->rtc_cols_var & 3) + mr + ">"};
     const int pnt = rtc_threads(2LL * h->N);
-    if (pair)
+    // only the passes this handle runs (slot: 0 rows fwd, 1 rows inv, 2 cols,
+    // 3 filtered sinogram pair)
+    std::vector<std::string> names;
+    std::vector<int> slot;
+    if (rows) {
+        const std::string args = std::to_string(h->h) + ", " + std::to_string(R) + ", " + std::to_string(ldm) + ", " +
+                                 std::to_string(rnt) + ", " + wpe + ", 1" + hr + ">";
+        names.push_back("lpct::k_rows_fwd_ct<" + args);
+        slot.push_back(0);
+        names.push_back("lpct::k_rows_inv_ct<" + args);
+        slot.push_back(1);
+    }
+    if (cols) {
+        names.push_back("lpct::k_cols_ct<" + std::to_string(h->Nr) + ", " + std::to_string(cnt) + ", " + wpe + ", " +
+                        std::to_string(h->rtc_cols_var & 3) + mr + ">");
+        slot.push_back(2);
+    }
+    if (pair) {
         names.push_back("lpct::k_sino_rows_pair_ct<" + std::to_string(h->N) + ", " + std::to_string(pnt) + ", " + wpe +
                         radix_list(h->plan_2N) + ">");
+        slot.push_back(3);
+    }
     std::string err;
     const int pp = (h->rtc_cols_var & 8) ? 4 : 8;
     auto code = rtc_compile(arch, pp, names, err);
@@ -895,7 +911,7 @@ void rtc_build(lp_handle* h) {
         mod = nullptr;
     }
     void* fn[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int i = 0; mod && i < (int)names.size(); ++i) {
+    for (size_t i = 0; mod && i < names.size(); ++i) {
         hipFunction_t f = nullptr;
         if (hipModuleGetFunction(&f, mod, code->lowered[i].c_str()) != hipSuccess) {
             (void)hipGetLastError();
@@ -904,25 +920,20 @@ void rtc_build(lp_handle* h) {
             mod = nullptr;
             break;
         }
-        fn[i] = (void*)f;
+        fn[slot[i]] = (void*)f;
     }
     if (!mod) {
         if (verbose) std::fprintf(stderr, "lprec_amd: run-time compiled passes unavailable (%s)\n", err.c_str());
         return;
     }
     h->rtc_mod = mod;
-    // a pass whose length the radix-16 kernels cover keeps them
-    h->rtc_fn[0] = rows ? fn[0] : nullptr;
-    h->rtc_fn[1] = rows ? fn[1] : nullptr;
-    h->rtc_fn[2] = cols ? fn[2] : nullptr;
-    h->rtc_fn[3] = pair ? fn[3] : nullptr;
+    for (int i = 0; i < 4; ++i) h->rtc_fn[i] = fn[i];
     h->rtc_R = R;
     h->rtc_rows_nt = rnt;
     h->rtc_cols_nt = cnt;
     h->rtc_pair_nt = pnt;
     if (verbose)
-        for (size_t i = 0; i < names.size(); ++i)
-            if (h->rtc_fn[i]) std::fprintf(stderr, "lprec_amd: run-time compiled pass: %s\n", names[i].c_str());
+        for (const auto& n : names) std::fprintf(stderr, "lprec_amd: run-time compiled pass: %s\n", n.c_str());
 }
 
 }  // namespace

@@ -1680,15 +1680,20 @@ void launch_rows_fwd(const RowPass& rp0, const float* F, const int2* range, hipS
     // the mixed-radix kernels keep at most 8 rows per block (their register
     // budget); Nrho stays divisible since R is a power of two
     RowPass rg = rp;
-    rg.R = std::min(rg.R, 8);
+    // the compiled pass has its rows per block built in
+    rg.R = rg.rtc_rows_fwd ? rg.rtc_R : std::min(rg.R, 8);
     rg = fit_zgroup(rg, 3LL * (rg.Nrho / rg.R));
     dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
-    if (rg.rtc_rows_fwd && rg.R == rg.rtc_R) {
+    if (rg.rtc_rows_fwd) {
         void* args[] = {&rg, &F, &range};
         if (hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_fwd, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
                                   args, nullptr) == hipSuccess)
             return;
-        (void)hipGetLastError();   // the generic kernel below does the pass
+        (void)hipGetLastError();   // the generic kernel does the pass
+        RowPass rn = rp;
+        rn.rtc_rows_fwd = nullptr;
+        launch_rows_fwd(rn, F, range, s);
+        return;
     }
     int nt = threads_for((long long)rg.R * rg.h);
     int ept = ept_for((long long)rg.R * rg.h, nt);
@@ -1723,15 +1728,19 @@ void launch_rows_inv(const RowPass& rp0, float* G, const int2* wrange, hipStream
         return;
     }
     RowPass rg = rp;
-    rg.R = std::min(rg.R, 8);
+    rg.R = rg.rtc_rows_inv ? rg.rtc_R : std::min(rg.R, 8);
     rg = fit_zgroup(rg, 3LL * (rg.Nrho / rg.R));
     dim3 grid(rg.Nrho / rg.R, 3, (rg.nz + rg.zgroup - 1) / rg.zgroup);
-    if (rg.rtc_rows_inv && rg.R == rg.rtc_R) {
+    if (rg.rtc_rows_inv) {
         void* args[] = {&rg, &G, &wrange};
         if (hipModuleLaunchKernel((hipFunction_t)rg.rtc_rows_inv, grid.x, grid.y, grid.z, rg.rtc_rows_nt, 1, 1, 0, s,
                                   args, nullptr) == hipSuccess)
             return;
-        (void)hipGetLastError();   // the generic kernel below does the pass
+        (void)hipGetLastError();   // the generic kernel does the pass
+        RowPass rn = rp;
+        rn.rtc_rows_inv = nullptr;
+        launch_rows_inv(rn, G, wrange, s);
+        return;
     }
     int nt = threads_for((long long)rg.R * rg.h);
     int ept = ept_for((long long)rg.R * rg.h, nt);

@@ -226,8 +226,9 @@ def test_headline_batch_matches_one_slice_handle(headline, torch_cuda):
     assert worst <= 1e-6
 
 
-@pytest.mark.parametrize("N0,P0,ns,pick", [(1024, 1024, 16, (0, 7, 15)), (2048, 2048, 4, (0, 3))],
-                         ids=["N1024_P1024", "N2048_P2048"])
+@pytest.mark.parametrize("N0,P0,ns,pick", [(1024, 1024, 16, (0, 7, 15)), (2048, 2048, 4, (0, 3)),
+                                            (4096, 1500, 2, (1,))],
+                         ids=["N1024_P1024", "N2048_P2048", "N4096_P1500"])
 def test_large_geometries_match_mirror(torch_cuda, N0, P0, ns, pick):
     torch = torch_cuda
     print("geometry", N0, P0, ns, flush=True)

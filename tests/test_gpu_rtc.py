@@ -29,6 +29,7 @@ CASES = [
     (250, 300, 2, "hamming", 127.3, "linear"), # N = 256: h = 128 is below the radix-16 row kernels' range
     (1024, 360, 2, "none", 512, "cubic"),      # N = 1024: the radix-16 kernels, no run-time passes
     (1024, 360, 2, "shepp-logan", 512, "cubic"),  # N = 1024 with a filter: the compiled sinogram pair pass only
+    (4096, 360, 1, "parzen", 2048, "cubic"),   # N = 4096: rho 8192 (above the radix-16 column kernels), filter
 ]
 
 
