@@ -34,6 +34,7 @@ CASES = [
     (512, 384, 2, "hamming", 256, "cubic"),    # fixture geometry, os = 2
     (528, 401, 1, "parzen", 261, "cubic"),     # N = 528 = 2^4 3 11 (generic radix), odd P0
     (224, 34, 2, "none", 112, "cubic"),        # os = 10 and 224 = 2^5 7: the foam fixture's regime
+    (64, 48, 3, "ramp", 32, "cubic"),          # smallest grids: theta half length 32, rho 128
 ]
 
 
