@@ -73,7 +73,7 @@ _STAGE_OF_KERNEL = (
     ("k_gather1<0", "gather_lp"), ("k_gather1<1", "gather_lp"), ("k_add_extras", "gather_lp"),
     ("k_gather1<2", "polar_out"), ("k_gather<3", "cart_out"),
     ("k_rows16_fwd", "rows_fwd"), ("k_rows_fwd", "rows_fwd"), ("k_cols16", "cols"), ("k_cols", "cols"),
-    ("k_rows16_inv", "rows_inv"), ("k_rows_inv", "rows_inv"),
+    ("k_rows16_inv", "rows_inv"), ("k_rows32_inv", "rows_inv"), ("k_rows_inv", "rows_inv"),
 )
 
 

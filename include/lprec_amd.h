@@ -111,8 +111,10 @@ int lp_info(const lp_handle* h, int64_t* info);
  * while a pass would have fewer than 2048 blocks), key 2 = rows per
  * prefilter column chunk (32, 64 or 128), key 3 = slices per block in the
  * tiled gathers (>=1), key 4 = FFT kernel family (1 = radix-16 register
- * kernels where the length is 16*k*16 with k in {1,2,4,8,16}, 0 = the
- * mixed-radix LDS kernels everywhere), key 5 = LDS budget of a gather block
+ * kernels where the length is 16*k*16 with k in {1,2,4,8,16}; 2, the
+ * default = the same with the inverse row pass at h = 1024 as a 32 x 32
+ * register FFT with 32 lanes per row; 0 = the mixed-radix LDS kernels
+ * everywhere), key 5 = LDS budget of a gather block
  * in floats (1024..32768; tiles are split until their source box fits; only
  * before lp_init_fwd / lp_init_adj), key 6 = forward operator's inverse row
  * pass and polar gather fused into one kernel (1) or run as two (0, the

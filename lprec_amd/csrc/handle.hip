@@ -70,7 +70,7 @@ struct lp_handle {
     // rows 13 b - 1, so each reads two 128-byte lines per spectrum column
     // where the separate pass reads one, and it measured no faster than the
     // separate pair (1.91 vs 1.90 ms per 32 slices at N = 2048)
-    int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 1, fuse_polar = 0;
+    int R = 16, zgroup = 8, colchunk = 32, gzgroup = 4, fft16 = 2, fuse_polar = 0;
     int polar_t = 1;   // separable polar transpose where the geometry allows (else the tiled scatter)
     // slices per block of the log-polar -> polar gather: its coordinate table
     // ([P][N] float2) is read once per block.  8 measured 0.587 against 0.595
@@ -1737,7 +1737,7 @@ int lp_set_tuning(lp_handle* h, int key, int value) {
             h->gzgroup = value;
             return LP_OK;
         case 4:
-            if (value != 0 && value != 1) return h->fail(LP_ERR_ARG, "FFT kernel family must be 0 or 1");
+            if (value < 0 || value > 2) return h->fail(LP_ERR_ARG, "FFT kernel family must be 0, 1 or 2");
             h->fft16 = value;
             return LP_OK;
         case 6:

@@ -1432,6 +1432,140 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
     }
 }
 
+// Length-32 DFT in registers, natural order in and out: one radix-2 step
+// over two register radix-16 DFTs (even and odd elements), twiddles
+// W32^k = exp(dir 2 pi i k / 32) as constants.
+__device__ __forceinline__ void dft32(float2* v, int dir) {
+    // cos and sin of pi k / 16
+    constexpr float C[16] = {1.0f, 0.98078528040323044913f, 0.92387953251128675613f, 0.83146961230254523708f,
+                             0.70710678118654752440f, 0.55557023301960222474f, 0.38268343236508977173f,
+                             0.19509032201612826785f, 0.0f, -0.19509032201612826785f, -0.38268343236508977173f,
+                             -0.55557023301960222474f, -0.70710678118654752440f, -0.83146961230254523708f,
+                             -0.92387953251128675613f, -0.98078528040323044913f};
+    constexpr float S[16] = {0.0f, 0.19509032201612826785f, 0.38268343236508977173f, 0.55557023301960222474f,
+                             0.70710678118654752440f, 0.83146961230254523708f, 0.92387953251128675613f,
+                             0.98078528040323044913f, 1.0f, 0.98078528040323044913f, 0.92387953251128675613f,
+                             0.83146961230254523708f, 0.70710678118654752440f, 0.55557023301960222474f,
+                             0.38268343236508977173f, 0.19509032201612826785f};
+    float2 e[16], o[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        e[k] = v[2 * k];
+        o[k] = v[2 * k + 1];
+    }
+    dft16(e, dir);
+    dft16(o, dir);
+    const float d = (float)dir;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        float2 t;
+        if (k == 0) t = o[0];
+        else if (k == 8) t = rot90(o[8], dir);
+        else t = cmul(o[k], make_float2(C[k], d * S[k]));
+        v[k] = cadd(e[k], t);
+        v[k + 16] = csub(e[k], t);
+    }
+}
+
+// Inverse row pass at h = 1024 (Ntheta = 2048) with 32 lanes per row: a lane
+// holds 32 elements, so the length-1024 FFT is one register DFT32 over
+// elements 32 apart, the twiddles W1024^(l m), one LDS exchange within the
+// row and a second register DFT32 (1024 = 32 x 32).  16 rows per 512-lane
+// block (the transposed half-spectra are read in 128-byte segments); at 512
+// lanes a lane has 256 VGPRs, which hold the next slice's 32 loads while the
+// current slice is transformed and stored.  (The same scheme for the forward
+// row pass measured slower than k_rows16_fwd, with or without the prefetch.)
+// LDS: 16 rows at stride 1090 slots (the layout of k_rows16_*), then the
+// twiddles W1024^(l m) at [m * 32 + l] (a fixed m is read conflict-free by
+// the 32 lanes l) and twN[0, 512].
+constexpr int kRows32RS = 1024 + 1024 / 16 + 2;
+constexpr size_t kRows32Lds = (size_t)(16 * kRows32RS + 1024 + 513) * sizeof(float2);
+
+__device__ __forceinline__ void rows32_tables(float2* tw2, float2* twn, const RowPass& rp) {
+    for (int i = threadIdx.x; i < 1024; i += 512) tw2[i] = rp.twh[(i & 31) * (i >> 5)];
+    for (int i = threadIdx.x; i <= 512; i += 512) twn[i] = rp.twN[i];
+}
+
+// Length-1024 FFT of the row owned by lanes li = 0..31 of one wave half,
+// v[q] = x[li + 32 q] on entry, v[k] = X[li + 32 k] on exit.  The exchange
+// slots li * 33 + q of the row (odd stride: conflict-free both ways) are
+// free on entry and free again on exit.
+__device__ __forceinline__ void rows32_fft(float2* v, float2* row, const float2* tw2, int li, int dir) {
+    dft32(v, dir);
+#pragma unroll
+    for (int m = 1; m < 32; ++m) {
+        const float2 w = tw2[m * 32 + li];
+        v[m] = cmul(v[m], dir < 0 ? w : cconj(w));
+    }
+#pragma unroll
+    for (int m = 0; m < 32; ++m) row[m * 33 + li] = v[m];
+    wave_sync();
+#pragma unroll
+    for (int l = 0; l < 32; ++l) v[l] = row[li * 33 + l];
+    dft32(v, dir);
+    wave_sync();   // the exchange slots are read before the caller reuses them
+}
+
+__global__ void __launch_bounds__(512) k_rows32_inv(RowPass rp, float* __restrict__ G,
+                                                    const int2* __restrict__ wrange) {
+    extern __shared__ float2 smc[];
+    constexpr int H = 1024, T = 32, R = 16, RS = kRows32RS;
+    const int Nt = rp.Ntheta, Nr = rp.Nrho;
+    const int r = threadIdx.x >> 5, li = threadIdx.x & 31;
+    const int nrb = Nr / R, nzg = (rp.nz + rp.zgroup - 1) / rp.zgroup;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int row0 = (lb % nrb) * R, span = lb / (nrb * nzg);
+    const int z0 = ((lb / nrb) % nzg) * rp.zgroup;
+    const int z1 = min(z0 + rp.zgroup, rp.nz);
+    const int2 wr = wrange ? wrange[(size_t)span * Nr + row0 + r] : make_int2(0, Nt);
+    const int e0 = max(wr.x, 0) >> 1, e1 = min((wr.y + 1) >> 1, H);
+    if (rp.out_rows && !__syncthreads_or(e0 < e1)) return;
+    float2* row = smc + r * RS;
+    float2* tw2 = smc + R * RS;
+    float2* twn = tw2 + 1024;
+    rows32_tables(tw2, twn, rp);
+    // transposed load: thread (rr, f0) = (tid mod 16, tid / 16) loads
+    // frequencies f0 + 32 i of row rr
+    const int rr = threadIdx.x & (R - 1), f0 = threadIdx.x >> 4;
+    const unsigned lane = (unsigned)(f0 * Nr + rr) * 8u;
+    float2* dst = smc + rr * RS + lidx(f0);
+    const unsigned step = 32u * (unsigned)Nr * 8u;   // bytes between frequencies f0 + 32 i and f0 + 32 (i + 1)
+    auto load = [&](int z, float2* w) {
+        const float2* in = rp.S + ((size_t)span * rp.nz + z) * H * (size_t)Nr + row0;
+        unsigned off = lane;
+        asm volatile("" : "+v"(off));
+#pragma unroll
+        for (int i = 0; i < 32; ++i) w[i] = ld_at(in, off + (unsigned)i * step);
+    };
+    float2 v[32], nx[32];
+    load(z0, nx);
+    for (int z = z0; z < z1; ++z) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) dst[34 * i] = nx[i];
+        lds_barrier();   // every row was written by the whole block (and the tables, first time)
+        // the lane index, opaque per slice: the LDS addresses derived from it
+        // are recomputed here instead of being held across the loop next to
+        // the prefetched loads
+        int l = li;
+        asm volatile("" : "+v"(l));
+        c2r_pre_row<H, T>(row, twn, l);
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < 32; ++q) v[q] = row[lidx(l + 32 * q)];
+        wave_sync();   // the exchange slots overlap the row
+        if (z + 1 < z1) load(z + 1, nx);   // in flight while this slice is transformed and stored
+        rows32_fft(v, row, tw2, l, +1);
+        // v[k] = x[li + 32 k]: only the columns the output gather reads
+        float2* out = reinterpret_cast<float2*>(G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt));
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int e = l + 32 * k;
+            if (e >= e0 && e < e1) out[e] = v[k];
+        }
+        lds_barrier();   // the next slice's staging overwrites every row
+    }
+}
+
 // Inverse row pass fused with the log-polar -> polar gather of the forward
 // operator.  A block owns a band of `nbase` = R - 3 base rows of one span:
 // it inverse-transforms the R rows [c0, c0 + R) (c0 = band * nbase - 1, mod
@@ -1643,6 +1777,8 @@ static int rows16_mid(const RowPass& rp) {
     if ((long long)rp.R * (h / 16) > 1024) return 0;
     return h / 256;
 }
+// the 32-lane inverse row kernel: FFT kernel family 2, h = 1024, 16 rows per block
+static bool rows32_applies(const RowPass& rp) { return rp.fft16 == 2 && rp.h == 1024 && rp.R == 16; }
 static size_t rows16_lds(const RowPass& rp) {
     // the rows, then (h <= 2048) the staged twiddles twh[0, h/2) and twN[0, h/2]
     const size_t tws = rp.h <= 2048 ? (size_t)(rp.h + 1) * sizeof(float2) : 0;
@@ -1723,6 +1859,11 @@ void launch_rows_inv(const RowPass& rp0, float* G, const int2* wrange, hipStream
     const RowPass rp = fit_zgroup(rp0, 3LL * (rp0.Nrho / rp0.R));
     if (int mid = rows16_mid(rp)) {
         dim3 grid((rp.Nrho / rp.R) * 3 * ((rp.nz + rp.zgroup - 1) / rp.zgroup));
+        if (rows32_applies(rp)) {
+            allow_lds(k_rows32_inv, kRows32Lds);
+            hipLaunchKernelGGL(k_rows32_inv, grid, dim3(512), kRows32Lds, s, rp, G, wrange);
+            return;
+        }
         dim3 block(rp.R * (rp.h / 16));
         LP_DISPATCH_MID(mid, k_rows16_inv, grid, block, rows16_lds(rp), s, rp, G, wrange);
         return;

@@ -1,9 +1,10 @@
 """Schedule autotuner for one operator handle.
 
-The kernels expose six schedule knobs through ``lp_set_tuning`` (log-polar
-rows per row-FFT block, slices per FFT block, prefilter column chunk, slices
-per gather block, FFT kernel family, the forward operator's fused inverse row
-pass + polar gather).  ``autotune`` times fwd+adj (and,
+The kernels expose seven schedule knobs through ``lp_set_tuning`` that can
+change after the tables are built (log-polar rows per row-FFT block, slices
+per FFT block, prefilter column chunk, slices per gather block, FFT kernel
+family, the forward operator's fused inverse row pass + polar gather, slices
+per block of the polar output gather).  ``autotune`` times fwd+adj (and,
 optionally, the transposes) on the handle's own stream with device events
 and walks the knobs one at a time (coordinate descent, two sweeps), keeping
 the fastest setting of each; settings the handle rejects (LDS or register
@@ -27,7 +28,7 @@ SPACE = {
     "gzgroup": (1, 2, 4, 8),
     "pzgroup": (4, 8, 16),
     "colchunk": (32, 64, 128),
-    "fft16": (1, 0),
+    "fft16": (2, 1, 0),
     "fuse": (1, 0),
 }
 

@@ -1,9 +1,10 @@
 """GPU parity at the sizes of the measured workloads (through the C ABI).
 
 The headline workload is N0 = 2048, 1500 angles, cor = 1024, cubic, a batch
-of 32 slices with the default schedule: the register radix-16 FFT kernels
-(rows of h = 1024 complex points, columns of Nrho = 4096) and the slice
-grouping of the gathers and FFT passes.  Checks:
+of 32 slices with the default schedule: the register FFT kernels (forward
+rows of h = 1024 complex points in radix 16, inverse rows as 32 x 32 with 32
+lanes per row, columns of Nrho = 4096) and the slice grouping of the gathers
+and FFT passes.  Checks:
 
 * stage by stage on the 32-slice call (lp_debug_stop): each stage's device
   output against the FP64 restatement of that stage applied to the device's
@@ -180,6 +181,62 @@ def test_headline_stages_match_restated_stages(headline, adjoint):
     e = stage_check(lp, tab, 32, (0, 13, 31), adjoint, seed=11 + adjoint)
     print("stage errors: prefilter %.2e gather %.2e rowfft %.2e columns %.2e convolution %.2e" % e, flush=True)
     assert e[0] < 2e-6 and e[1] < 5e-6 and e[2] < 2e-6 and e[3] < 5e-6 and e[4] < 1e-5
+
+
+@pytest.mark.parametrize("adjoint", [False, True], ids=["fwd", "adj"])
+def test_radix16_inverse_rows_match_restated_stages(headline, adjoint):
+    """FFT kernel family 1 (key 4): the radix-16 inverse row kernel
+    (k_rows16_inv<4>) in place of the default 32-lane one (k_rows32_inv) --
+    the whole convolution (stage 6) against numpy."""
+    lp, tab = headline
+    h = lp.clphandle[0]
+    old = h.get_tuning(4)
+    try:
+        h.set_tuning(4, 1)
+        print("headline stages, radix-16 inverse rows,", "adj" if adjoint else "fwd", flush=True)
+        e = stage_check(lp, tab, 32, (0, 13, 31), adjoint, seed=21 + adjoint)
+    finally:
+        h.set_tuning(4, old)
+    print("stage errors: prefilter %.2e gather %.2e rowfft %.2e columns %.2e convolution %.2e" % e, flush=True)
+    assert e[2] < 2e-6 and e[3] < 5e-6 and e[4] < 1e-5
+
+
+def test_row_kernel_families_agree(headline, torch_cuda):
+    """The four operators with the 32-lane inverse row kernel (family 2, the
+    default) against the radix-16 one (family 1) on the 32-slice batch
+    (different FFT factorisations, so equal to rounding)."""
+    torch = torch_cuda
+    lp, tab = headline
+    h = lp.clphandle[0]
+    ns, N0, P0 = 32, 2048, 1500
+    rng = np.random.default_rng(32)
+    f = torch.from_numpy(rng.standard_normal((ns, N0, N0)).astype(np.float32)).cuda()
+    R = torch.from_numpy(rng.standard_normal((ns, P0, N0)).astype(np.float32)).cuda()
+    old = h.get_tuning(4)
+    outs = []
+    try:
+        for fam in (1, 2):
+            h.set_tuning(4, fam)
+            Ro = torch.zeros((ns, P0, N0), device="cuda")
+            fo = torch.zeros((ns, N0, N0), device="cuda")
+            fT = torch.zeros((ns, N0, N0), device="cuda")
+            RT = torch.zeros((ns, P0, N0), device="cuda")
+            lp.fwdp(Ro, f, 0)
+            lp.adjp(fo, R, 0)
+            lp.fwdp_transpose(fT, R, 0)
+            lp.adjp_transpose(RT, f, 0)
+            torch.cuda.synchronize()
+            outs.append([t.cpu().numpy() for t in (Ro, fo, fT, RT)])
+            del Ro, fo, fT, RT
+    finally:
+        h.set_tuning(4, old)
+    worst = 0.0
+    for name, a, b in zip(("fwd", "adj", "fwd_T", "adj_T"), outs[1], outs[0]):
+        d = rel(a, b)
+        print("%s: 32-lane vs radix-16 inverse row kernel %.2e" % (name, d), flush=True)
+        worst = max(worst, d)
+    release()
+    assert worst <= 1e-6
 
 
 def test_headline_batch_matches_mirror(headline):
