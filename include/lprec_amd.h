@@ -139,8 +139,9 @@ int lp_info(const lp_handle* h, int64_t* info);
  * mask (bit 0: the column pass loads the next slice's column while the
  * current one is transformed; bit 1: its index arithmetic is redone per slice
  * instead of held in registers; bit 2: at least 4 waves per SIMD; bit 3:
- * odd-radix work items of 4 output pairs instead of 8; default 14, the
- * fastest measured at N = 2064; recompiles), key 13 = slices per block of
+ * odd-radix work items of 4, 6 or 8 output pairs, whichever takes the fewest
+ * rounds of the block's threads over the module's radix-R > 5 stages,
+ * instead of 8; default 14; recompiles), key 13 = slices per block of
  * lp_fwd's log-polar -> polar gather (>= 1, default 4; each block reads its
  * tile's coordinates once), key 14 = lp_fwd's image -> log-polar gather
  * reads every tap from global memory, one thread per point (1), instead of

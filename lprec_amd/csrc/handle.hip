@@ -3,6 +3,7 @@
 // lpRgpu (src/include/lpRgpu.cuh:6-81, src/cuda/lpRgpu.cu, params.cu,
 // gridStorage.cu) behind the C ABI of include/lprec_amd.h.
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <complex>
@@ -89,6 +90,7 @@ struct lp_handle {
     // variant of the compiled passes: k_cols_ct VAR (bits 0-1), at least 4
     // waves per SIMD (bit 2), odd-radix output pairs per item 4 instead of 8 (bit 3)
     int rtc_cols_var = 14;
+    int rtc_pp = 0;   // output pairs per odd-radix work item of the loaded module
     bool rtc_tried = false;
     hipModule_t rtc_mod = nullptr;
     void* rtc_fn[4] = {nullptr, nullptr, nullptr, nullptr};   // + the filtered sinogram pair pass
@@ -844,6 +846,34 @@ std::shared_ptr<RtcCode> rtc_compile(const std::string& arch, int pp, const std:
 
 bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 
+// Output pairs per odd-radix work item (LPCT_ODD_PP of fft_ct.hpp, one value
+// per compiled module) from 4, 6 and 8: the fewest cycles over the module's
+// passes, counting for each radix R > 5 the rounds of the block's threads
+// over its work items (lines x M/R butterflies x ceil(hR/PP) groups) times
+// the pairs per item (hR = (R-1)/2 terms each).  E.g. the 4128-point column
+// at 512 threads: PP 4 -> 576 items, two rounds of 4; PP 6 -> 384, one of 6.
+int odd_pairs(const std::vector<std::array<int, 4>>& passes, const std::vector<const FftPlan*>& plans) {
+    int best = 4;
+    long long best_cost = LLONG_MAX;
+    for (int pp : {4, 6, 8}) {
+        long long cost = 0;
+        for (size_t k = 0; k < passes.size(); ++k) {
+            const int M = passes[k][0], NL = passes[k][1], NT = passes[k][2], weight = passes[k][3];
+            const FftPlan& pl = *plans[k];
+            for (int i = 0; i < pl.nst; ++i) {
+                const int R = pl.radix[i];
+                if (R <= 5 || !(R & 1)) continue;
+                const int hR = (R - 1) / 2;
+                const int P = hR <= 2 ? 2 : (hR <= pp ? ((hR + 1) & ~1) : pp);
+                const long long items = (long long)NL * (M / R) * ((hR + P - 1) / P);
+                cost += weight * ((items + NT - 1) / NT) * P * hR;
+            }
+        }
+        if (cost < best_cost) { best_cost = cost; best = pp; }
+    }
+    return best;
+}
+
 // threads of a mixed-radix pass over `points` values (as threads_for)
 int rtc_threads(long long points) {
     int nt = 256;
@@ -901,8 +931,24 @@ void rtc_build(lp_handle* h) {
                         radix_list(h->plan_2N) + ">");
         slot.push_back(3);
     }
+    // the passes of the module (length, lines per block, threads, launches
+    // per fwd or adj call) for the choice of output pairs per work item
+    std::vector<std::array<int, 4>> passes;
+    std::vector<const FftPlan*> plans;
+    if (rows) {
+        passes.push_back({h->h, R, rnt, 2});
+        plans.push_back(&h->plan_h);
+    }
+    if (cols) {
+        passes.push_back({h->Nr, 1, cnt, 1});
+        plans.push_back(&h->plan_M);
+    }
+    if (pair) {
+        passes.push_back({2 * h->N, 1, pnt, 1});
+        plans.push_back(&h->plan_2N);
+    }
     std::string err;
-    const int pp = (h->rtc_cols_var & 8) ? 4 : 8;
+    const int pp = (h->rtc_cols_var & 8) ? odd_pairs(passes, plans) : 8;
     auto code = rtc_compile(arch, pp, names, err);
     hipModule_t mod = nullptr;
     if (code && hipModuleLoadData(&mod, code->code.data()) != hipSuccess) {
@@ -932,8 +978,10 @@ void rtc_build(lp_handle* h) {
     h->rtc_rows_nt = rnt;
     h->rtc_cols_nt = cnt;
     h->rtc_pair_nt = pnt;
+    h->rtc_pp = pp;
     if (verbose)
-        for (const auto& n : names) std::fprintf(stderr, "lprec_amd: run-time compiled pass: %s\n", n.c_str());
+        for (const auto& n : names)
+            std::fprintf(stderr, "lprec_amd: run-time compiled pass: %s (odd-radix output pairs %d)\n", n.c_str(), pp);
 }
 
 }  // namespace
