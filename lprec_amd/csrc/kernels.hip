@@ -94,7 +94,12 @@ __device__ __forceinline__ void st_at(T* base, unsigned off, T v) {
 // The same for float2 with the non-temporal hint (global_load/store ... nt):
 // data that is streamed once and not re-read from the caches.  Measured on
 // the inverse row pass (its S reads and G writes): 1.12 / 0.98 against
-// 1.19 / 1.05 ms; on the forward row pass (F reads, S writes) it was slower.
+// 1.19 / 1.05 ms; on the column pass at Nrho = 4096 (S in through
+// global_load_lds with the hint, kCpolNT, and S out): 1.26 / 1.28 against
+// 1.33 / 1.35 ms, neither half alone faster; on the radix-16 inverse row
+// pass at h = 512: 0.167 / 0.135 against 0.177 / 0.141 ms.  Slower on the
+// forward row pass (F reads, S writes); no gain on k_cols16.
+constexpr int kCpolNT = 2;   // cache-policy operand of the load-to-LDS builtin: nt
 typedef float v2f_nt __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 ld_at_nt(const float2* base, unsigned off) {
     const v2f_nt v =
@@ -1076,7 +1081,7 @@ __global__ void __launch_bounds__(256, 2) k_cols4096(RowPass rp, const float2* _
         for (int c = 0; c < M / 128 / 4; ++c) {
             const int cc = wv + 4 * c;
             if (cc >= c_lo && cc < c_hi)
-                __builtin_amdgcn_global_load_lds(col + cc * 128 + ln * 2, pre + cc * 128, 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(col + cc * 128 + ln * 2, pre + cc * 128, 16, 0, kCpolNT);
         }
     };
     prefetch(z0);
@@ -1145,7 +1150,7 @@ __global__ void __launch_bounds__(256, 2) k_cols4096(RowPass rp, const float2* _
         dft16(v, +1);
 #pragma unroll
         for (int q = 0; q < 16; ++q)
-            if (smask & (1u << q)) col[li + q * T16] = v[q];
+            if (smask & (1u << q)) st_at_nt(col, (unsigned)(li + q * T16) * 8u, v[q]);
         lds_barrier();
     }
 }
@@ -1415,7 +1420,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
             const unsigned lane = (unsigned)(f0 * Nr + rr) * 8u;
             float2 w[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) w[i] = ld_at(in + (size_t)(i * T16) * Nr, lane);
+            for (int i = 0; i < 16; ++i) w[i] = ld_at_nt(in + (size_t)(i * T16) * Nr, lane);
 #pragma unroll
             for (int i = 0; i < 16; ++i) dst[i * (T16 + T16 / 16)] = w[i];
         }
@@ -1441,7 +1446,7 @@ __global__ void __launch_bounds__(1024) k_rows16_inv(RowPass rp, float* __restri
         for (int q = 0; q < 16; ++q) rowbuf[lidx(li + q * T16)] = v[q];
         row_sync<WAVE>();
         float2* out = reinterpret_cast<float2*>(G + ((((size_t)span * rp.nz + z) * Nr + row0 + r) * (size_t)Nt));
-        for (int e = e0 + li; e < e1; e += T16) out[e] = rowbuf[lidx(e)];
+        for (int e = e0 + li; e < e1; e += T16) st_at_nt(out, (unsigned)e * 8u, rowbuf[lidx(e)]);
         lds_barrier();   // the next slice's staging overwrites every row (stores drain meanwhile)
     }
 }
