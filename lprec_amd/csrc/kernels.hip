@@ -1493,8 +1493,8 @@ __device__ __forceinline__ void dft32(float2* v, int dir) {
 // block (the transposed half-spectra are read in 128-byte segments); at 512
 // lanes a lane has 256 VGPRs, which hold the next slice's 32 loads while the
 // current slice is transformed and stored.  S is read and G written with the
-// non-temporal hint.  (The same scheme for the forward
-// row pass measured slower than k_rows16_fwd, with or without the prefetch.)
+// non-temporal hint.  (The same scheme for the forward row pass measured
+// slower than k_rows16_fwd, with or without the prefetch.)
 // LDS: 16 rows at stride 1090 slots (the layout of k_rows16_*), then the
 // twiddles W1024^(l m) at [m * 32 + l] (a fixed m is read conflict-free by
 // the 32 lanes l) and twN[0, 512].
@@ -1508,8 +1508,9 @@ __device__ __forceinline__ void rows32_tables(float2* tw2, float2* twn, const Ro
 
 // Length-1024 FFT of the row owned by lanes li = 0..31 of one wave half,
 // v[q] = x[li + 32 q] on entry, v[k] = X[li + 32 k] on exit.  The exchange
-// slots li * 33 + q of the row (odd stride: conflict-free both ways) are
-// free on entry and free again on exit.
+// goes through the row's slots m * 33 + l (m, l < 32; the odd stride keeps
+// both the writes, fixed m, and the reads, fixed l, free of bank
+// conflicts), which must be free on entry and are free again on exit.
 __device__ __forceinline__ void rows32_fft(float2* v, float2* row, const float2* tw2, int li, int dir) {
     dft32(v, dir);
 #pragma unroll
